@@ -1,0 +1,745 @@
+// bvh_build.hip -- device LBVH construction for gfx950.
+//
+// Replaces the reference's single-threaded host builder LBVH_build
+// (src/renderer/bvh.h:186-221) with an all-device pipeline that produces the
+// same tree:
+//   1. centroid bounds            (bvh.h:192-196)           two-level block reduction
+//   2. 30-bit Morton keys         (bvh.h:12-32, 199-204)    one thread per triangle
+//   3. (key, index) sort          (bvh.h:207-208)           LDS-staged MSD radix sort
+//   4. split weights              (bvh.h:161-176)           adjacent-pair common prefix;
+//      equal-key runs get 32 + the split depth of the reference's midpoint
+//      rule ((l+r)>>1)-1, so the radix tree reproduces that rule exactly
+//   5. tree + refit               (bvh.h:143-184)           bottom-up agglomerative build
+//      (parent = deeper of the two boundary splits) fused with the box union
+//   6. layouts                    (bvh.h:121-141)           reference DFS order via
+//      pre-order index = 2*first_leaf + number_of_left_steps, and packed 128-B
+//      BVH4 nodes (binary nodes at even depth) for the traversal kernels.
+#include "dev_math.h"
+#include "gpr_internal.h"
+
+#include <algorithm>
+
+namespace gpr {
+namespace {
+
+constexpr int kB = 256;              // threads per block
+constexpr int kTileKeys = 4096;      // keys per sort tile (16 per thread)
+constexpr int kDigits = 1024;        // top 10 of the 30 key bits
+constexpr int kDigitShift = 20;
+constexpr int kLocalCap = 4096;      // bucket size sorted entirely in LDS
+constexpr int kMaxDepth = 256;       // > 30 radix levels + 32 midpoint levels
+
+// Load through the device-coherent path (another XCD's L2 may hold the line).
+template <class T>
+GPR_D T ld_agent(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Triangle::worldBound (src/renderer/triangle.h:96-101): Union(AABB(p0,p1), p2)
+GPR_D void tri_bounds(const float* p, v3& lo, v3& hi) {
+    const v3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
+    const v3 a = vmin3(p0, p1), b = vmax3(p0, p1);
+    lo = vmin3(a, p2);
+    hi = vmax3(b, p2);
+}
+
+// ------------------------------------------------------------ bounds ----
+template <int NT>
+GPR_D void block_minmax(v3& lo, v3& hi, float* sh) {
+    const int t = threadIdx.x;
+    sh[t * 6 + 0] = lo.x; sh[t * 6 + 1] = lo.y; sh[t * 6 + 2] = lo.z;
+    sh[t * 6 + 3] = hi.x; sh[t * 6 + 4] = hi.y; sh[t * 6 + 5] = hi.z;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            for (int k = 0; k < 3; ++k) sh[t * 6 + k] = fmin_h(sh[t * 6 + k], sh[(t + s) * 6 + k]);
+            for (int k = 3; k < 6; ++k) sh[t * 6 + k] = fmax_h(sh[t * 6 + k], sh[(t + s) * 6 + k]);
+        }
+        __syncthreads();
+    }
+    lo = V(sh[0], sh[1], sh[2]);
+    hi = V(sh[3], sh[4], sh[5]);
+}
+
+__global__ __launch_bounds__(kB) void k_centroid_partial(const float* __restrict__ pos, uint32_t n,
+                                                         float* __restrict__ part) {
+    __shared__ float sh[kB * 6];
+    v3 lo = V(INFINITY, INFINITY, INFINITY), hi = V(-INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+        v3 a, b;
+        tri_bounds(pos + 9ull * i, a, b);
+        const v3 c = (a + b) * 0.5f;  // AABB::centroid (aabb.h:23)
+        lo = vmin3(lo, c);
+        hi = vmax3(hi, c);
+    }
+    block_minmax<kB>(lo, hi, sh);
+    if (threadIdx.x == 0) {
+        float* o = part + 6 * blockIdx.x;
+        o[0] = lo.x; o[1] = lo.y; o[2] = lo.z; o[3] = hi.x; o[4] = hi.y; o[5] = hi.z;
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_centroid_final(const float* __restrict__ part, uint32_t np,
+                                                       float* __restrict__ out) {
+    __shared__ float sh[kB * 6];
+    v3 lo = V(INFINITY, INFINITY, INFINITY), hi = V(-INFINITY, -INFINITY, -INFINITY);
+    for (uint32_t i = threadIdx.x; i < np; i += kB) {
+        lo = vmin3(lo, load3(part + 6 * i));
+        hi = vmax3(hi, load3(part + 6 * i + 3));
+    }
+    block_minmax<kB>(lo, hi, sh);
+    if (threadIdx.x == 0) {
+        out[0] = lo.x; out[1] = lo.y; out[2] = lo.z; out[3] = hi.x; out[4] = hi.y; out[5] = hi.z;
+    }
+}
+
+// ------------------------------------------------------------ Morton ----
+// LeftShift3 (bvh.h:12-25): 10 bits to every third bit, 1024 -> 1023.
+GPR_D uint32_t spread3(uint32_t x) {
+    x = x == 1024u ? 1023u : x;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+// float -> uint truncation; NaN (flat axis, 0/0) -> 0.
+GPR_D uint32_t quant(float v) {
+    if (!(v == v) || v <= 0.0f) return 0u;
+    return v >= 4294967040.0f ? 0xffffffffu : (uint32_t)v;
+}
+
+__global__ __launch_bounds__(kB) void k_morton(const float* __restrict__ pos, uint32_t n,
+                                               const float* __restrict__ bounds,
+                                               uint32_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const v3 lo = load3(bounds), hi = load3(bounds + 3);
+    v3 a, b;
+    tri_bounds(pos + 9ull * i, a, b);
+    const v3 c = (a + b) * 0.5f;
+    const v3 q = (c - lo) / (hi - lo) * 1024.f;  // EncodeMorton3 input (bvh.h:203)
+    keys[i] = (spread3(quant(q.z)) << 2) | (spread3(quant(q.y)) << 1) | spread3(quant(q.x));
+}
+
+// -------------------------------------------------------------- sort ----
+// MSD radix sort specialised for 30-bit keys:
+//   hist    per-tile counts of the top 10 bits           (reads 4 B/key)
+//   scan    per-digit exclusive scan over tiles           (1 MB of counts at 1M keys)
+//   scatter keys into their digit bucket as (key<<32|idx)  (reads 4 B, writes 8 B/key)
+//   local   each bucket sorted as 64-bit composites in LDS (reads 8 B, writes 8 B/key)
+// Sorting the composite makes the result independent of the order inside a
+// bucket, so the scatter may use LDS atomics, and ties come out in index order.
+__global__ __launch_bounds__(kB) void k_sort_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                  uint32_t* __restrict__ counts) {
+    __shared__ uint32_t h[kDigits];
+    for (int d = threadIdx.x; d < kDigits; d += kB) h[d] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kTileKeys;
+#pragma unroll 4
+    for (int j = 0; j < kTileKeys / kB; ++j) {
+        const uint64_t i = base + j * kB + threadIdx.x;
+        if (i < n) atomicAdd(&h[keys[i] >> kDigitShift], 1u);
+    }
+    __syncthreads();
+    uint32_t* row = counts + (uint64_t)blockIdx.x * kDigits;
+    for (int d = threadIdx.x; d < kDigits; d += kB) row[d] = h[d];
+}
+
+// Inclusive block scan (256 threads, wave64) of one value per thread.
+GPR_D uint32_t block_incl_scan(uint32_t v, uint32_t* sh /* >= 4 */, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    if (lane == 63) sh[wave] = v;
+    __syncthreads();
+    uint32_t add = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kB / 64; ++w) {
+        const uint32_t s = sh[w];
+        if (w < wave) add += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return v + add;
+}
+
+__global__ __launch_bounds__(kB) void k_sort_scan(const uint32_t* __restrict__ counts, uint32_t ntiles,
+                                                  uint32_t* __restrict__ offs,
+                                                  uint32_t* __restrict__ totals) {
+    __shared__ uint32_t sh[4];
+    const uint32_t d = blockIdx.x;
+    uint32_t carry = 0;
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += kB) {
+        const uint32_t t = t0 + threadIdx.x;
+        const uint32_t c = t < ntiles ? counts[(uint64_t)t * kDigits + d] : 0u;
+        uint32_t tot;
+        const uint32_t inc = block_incl_scan(c, sh, &tot);
+        if (t < ntiles) offs[(uint64_t)t * kDigits + d] = carry + inc - c;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) totals[d] = carry;
+}
+
+// Exclusive scan of the 1024 digit totals into LDS (4 per thread).
+GPR_D void digit_bases(const uint32_t* __restrict__ totals, uint32_t* base, uint32_t* sh) {
+    const int t = threadIdx.x;
+    uint32_t v[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = totals[4 * t + k];
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_incl_scan(s, sh, &tot) - s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        base[4 * t + k] = run;
+        run += v[k];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kB) void k_sort_scatter(const uint32_t* __restrict__ keys, uint32_t n,
+                                                     const uint32_t* __restrict__ offs,
+                                                     const uint32_t* __restrict__ totals,
+                                                     unsigned long long* __restrict__ tmp) {
+    __shared__ uint32_t pos[kDigits];
+    __shared__ uint32_t sh[4];
+    digit_bases(totals, pos, sh);
+    const uint32_t* row = offs + (uint64_t)blockIdx.x * kDigits;
+    for (int d = threadIdx.x; d < kDigits; d += kB) pos[d] += row[d];
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kTileKeys;
+#pragma unroll 4
+    for (int j = 0; j < kTileKeys / kB; ++j) {
+        const uint64_t i = base + j * kB + threadIdx.x;
+        if (i < n) {
+            const uint32_t k = keys[i];
+            const uint32_t slot = atomicAdd(&pos[k >> kDigitShift], 1u);
+            tmp[slot] = ((unsigned long long)k << 32) | (unsigned long long)i;
+        }
+    }
+}
+
+GPR_D uint32_t lower_bound_u64(const unsigned long long* a, uint32_t n, unsigned long long v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Bitonic sort of P (power of two) 64-bit values in LDS.
+GPR_D void lds_bitonic(unsigned long long* s, uint32_t P) {
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < (P >> 1); i += kB) {
+                const uint32_t a = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                const uint32_t b = a + j;
+                const unsigned long long x = s[a], y = s[b];
+                const bool up = (a & k) == 0;
+                if ((x > y) == up) {
+                    s[a] = y;
+                    s[b] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(kB) void k_sort_local(const uint32_t* __restrict__ totals,
+                                                   unsigned long long* __restrict__ tmp,
+                                                   unsigned long long* __restrict__ alt,
+                                                   uint32_t* __restrict__ skeys,
+                                                   uint32_t* __restrict__ sidx) {
+    __shared__ unsigned long long s[kLocalCap];
+    __shared__ uint32_t red[kB];
+    const uint32_t d = blockIdx.x;
+    // bucket start = sum of totals[0..d)
+    uint32_t part = 0;
+    for (uint32_t i = threadIdx.x; i < d; i += kB) part += totals[i];
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int st = kB / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    const uint32_t b0 = red[0];
+    const uint32_t n = totals[d];
+    if (n == 0) return;
+    unsigned long long* src = tmp + b0;
+    if (n <= kLocalCap) {
+        uint32_t P = 2;
+        while (P < n) P <<= 1;
+        for (uint32_t i = threadIdx.x; i < P; i += kB) s[i] = i < n ? src[i] : ~0ull;
+        __syncthreads();
+        lds_bitonic(s, P);
+        for (uint32_t i = threadIdx.x; i < n; i += kB) {
+            const unsigned long long v = s[i];
+            skeys[b0 + i] = (uint32_t)(v >> 32);
+            sidx[b0 + i] = (uint32_t)v;
+        }
+        return;
+    }
+    // Oversized bucket (skewed key distributions): sort LDS-sized chunks, then
+    // merge runs pairwise through `alt`; each element finds its output rank by
+    // a binary search in the sibling run (composites are unique).
+    for (uint32_t c0 = 0; c0 < n; c0 += kLocalCap) {
+        const uint32_t m = min((uint32_t)kLocalCap, n - c0);
+        for (uint32_t i = threadIdx.x; i < kLocalCap; i += kB) s[i] = i < m ? src[c0 + i] : ~0ull;
+        __syncthreads();
+        lds_bitonic(s, kLocalCap);
+        for (uint32_t i = threadIdx.x; i < m; i += kB) src[c0 + i] = s[i];
+        __syncthreads();
+    }
+    unsigned long long* a = src;
+    unsigned long long* bb = alt + b0;
+    for (uint32_t w = kLocalCap; w < n; w <<= 1) {
+        for (uint32_t e = threadIdx.x; e < n; e += kB) {
+            const uint32_t r0 = (e / (2 * w)) * (2 * w);
+            const uint32_t mid = min(r0 + w, n), end = min(r0 + 2 * w, n);
+            const unsigned long long v = a[e];
+            uint32_t out;
+            if (e < mid) out = (e - r0) + lower_bound_u64(a + mid, end - mid, v);
+            else out = (e - mid) + lower_bound_u64(a + r0, mid - r0, v);
+            bb[r0 + out] = v;
+        }
+        __syncthreads();
+        unsigned long long* t = a;
+        a = bb;
+        bb = t;
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += kB) {
+        const unsigned long long v = a[i];
+        skeys[b0 + i] = (uint32_t)(v >> 32);
+        sidx[b0 + i] = (uint32_t)v;
+    }
+}
+
+// ---------------------------------------------------------- weights -----
+// Split depth of adjacent pair j in the reference's midpoint tree over a run
+// of m equal keys: root split after floor(m/2) leaves (bvh.h:162-165).
+GPR_D uint32_t midpoint_depth(uint32_t j, uint32_t m) {
+    uint32_t d = 0;
+    while (m > 1) {
+        const uint32_t h = m >> 1;
+        if (j == h - 1) return d;
+        if (j < h - 1) {
+            m = h;
+        } else {
+            j -= h;
+            m -= h;
+        }
+        ++d;
+    }
+    return d;
+}
+
+__global__ __launch_bounds__(kB) void k_weights(const uint32_t* __restrict__ sk, uint32_t n,
+                                                int32_t* __restrict__ w) {
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k + 1 >= n) return;
+    const uint32_t a = sk[k], b = sk[k + 1];
+    if (a != b) {
+        w[k] = __clz(a ^ b);
+        return;
+    }
+    // run [s, e) of key a
+    uint32_t lo = 0, hi = k;
+    while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (sk[m] < a) lo = m + 1;
+        else hi = m;
+    }
+    const uint32_t s = lo;
+    lo = k + 1;
+    hi = n;
+    while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (sk[m] <= a) lo = m + 1;
+        else hi = m;
+    }
+    w[k] = 32 + (int32_t)midpoint_depth(k - s, lo - s);
+}
+
+// ------------------------------------------------------ tree + refit ----
+// Node ids: internal node k (split between sorted leaves k and k+1) = k,
+// leaf i = (n-1)+i.  A node covering [l, r] has parent r (as left child) if
+// the split after r is deeper than the split before l, else parent l-1 (as
+// right child).  The second child to arrive at a parent forms its box.
+__global__ __launch_bounds__(kB) void k_tree(const float* __restrict__ pos, uint32_t n,
+                                             const uint32_t* __restrict__ sidx,
+                                             const int32_t* __restrict__ w,
+                                             uint32_t* __restrict__ parent,
+                                             uint32_t* __restrict__ child,
+                                             uint32_t* __restrict__ rlo, uint32_t* __restrict__ rhi,
+                                             uint32_t* __restrict__ arrive,
+                                             float* __restrict__ box,  // (2n-1)*6
+                                             uint32_t* __restrict__ root) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    v3 lo, hi;
+    tri_bounds(pos + 9ull * sidx[i], lo, hi);
+    uint32_t node = (n - 1) + i;
+    {
+        float* b = box + 6ull * node;
+        b[0] = lo.x; b[1] = lo.y; b[2] = lo.z; b[3] = hi.x; b[4] = hi.y; b[5] = hi.z;
+    }
+    uint32_t l = i, r = i;
+    for (int guard = 0; guard < kMaxDepth; ++guard) {
+        if (l == 0 && r == n - 1) {
+            *root = node;
+            break;
+        }
+        bool as_left;
+        if (l == 0) as_left = true;
+        else if (r == n - 1) as_left = false;
+        else as_left = w[r] > w[l - 1];
+        const uint32_t p = as_left ? r : l - 1;
+        parent[node] = p;
+        if (as_left) {
+            child[2 * p] = node;
+            atomicExch(&rlo[p], l);
+        } else {
+            child[2 * p + 1] = node;
+            atomicExch(&rhi[p], r);
+        }
+        __threadfence();
+        if (atomicAdd(&arrive[p], 1u) == 0) break;
+        __threadfence();
+        // second arrival: both children published
+        if (as_left) r = ld_agent(&rhi[p]);
+        else l = ld_agent(&rlo[p]);
+        const uint32_t lc = as_left ? node : ld_agent(&child[2 * p]);
+        const uint32_t rc = as_left ? ld_agent(&child[2 * p + 1]) : node;
+        const float* bl = box + 6ull * lc;
+        const float* br = box + 6ull * rc;
+        // Union(l_child, r_child) (aabb.h:37-46, bvh.h:182)
+        const v3 llo = V(ld_agent(bl + 0), ld_agent(bl + 1), ld_agent(bl + 2));
+        const v3 lhi = V(ld_agent(bl + 3), ld_agent(bl + 4), ld_agent(bl + 5));
+        const v3 rlo_ = V(ld_agent(br + 0), ld_agent(br + 1), ld_agent(br + 2));
+        const v3 rhi_ = V(ld_agent(br + 3), ld_agent(br + 4), ld_agent(br + 5));
+        lo = vmin3(llo, rlo_);
+        hi = vmax3(lhi, rhi_);
+        float* b = box + 6ull * p;
+        b[0] = lo.x; b[1] = lo.y; b[2] = lo.z; b[3] = hi.x; b[4] = hi.y; b[5] = hi.z;
+        node = p;
+    }
+}
+
+// Depth and number of left steps from the root, per node.
+__global__ __launch_bounds__(kB) void k_walkup(uint32_t n, const uint32_t* __restrict__ parent,
+                                               const uint32_t* __restrict__ child,
+                                               const uint32_t* __restrict__ rlo,
+                                               const uint32_t* __restrict__ root,
+                                               uint32_t* __restrict__ pre,
+                                               uint32_t* __restrict__ depth) {
+    const uint32_t x = blockIdx.x * kB + threadIdx.x;
+    if (x >= 2 * n - 1) return;
+    const uint32_t rt = *root;
+    uint32_t y = x, dep = 0, left = 0;
+    while (y != rt && dep < kMaxDepth) {
+        const uint32_t p = parent[y];
+        if (p >= n - 1) break;  // only reachable through a malformed tree
+        left += child[2 * p] == y ? 1u : 0u;
+        ++dep;
+        y = p;
+    }
+    const uint32_t first = x < n - 1 ? rlo[x] : x - (n - 1);
+    pre[x] = 2 * first + left;
+    depth[x] = dep;
+}
+
+__global__ __launch_bounds__(kB) void k_export_ref(uint32_t n, const uint32_t* __restrict__ child,
+                                                   const uint32_t* __restrict__ sidx,
+                                                   const float* __restrict__ box,
+                                                   const uint32_t* __restrict__ pre,
+                                                   gpr_ref_node* __restrict__ ref) {
+    const uint32_t x = blockIdx.x * kB + threadIdx.x;
+    if (x >= 2 * n - 1 || pre[x] >= 2 * n - 1) return;
+    gpr_ref_node o;
+    const float* b = box + 6ull * x;
+    o.bmin[0] = b[0]; o.bmin[1] = b[1]; o.bmin[2] = b[2];
+    o.bmax[0] = b[3]; o.bmax[1] = b[4]; o.bmax[2] = b[5];
+    if (x >= n - 1) {
+        o.idx = sidx[x - (n - 1)];
+        o.leaf = 1;
+    } else {
+        const uint32_t rc = child[2 * x + 1];
+        o.idx = rc < 2 * n - 1 ? pre[rc] : 0u;
+        o.leaf = 0;
+    }
+    ref[pre[x]] = o;
+}
+
+// Flags of BVH4 nodes (internal, even depth) in DFS (pre-order) position.
+__global__ __launch_bounds__(kB) void k_bvh4_flags(uint32_t n, const uint32_t* __restrict__ pre,
+                                                   const uint32_t* __restrict__ depth,
+                                                   uint32_t* __restrict__ flag) {
+    const uint32_t x = blockIdx.x * kB + threadIdx.x;
+    if (x >= 2 * n - 1 || pre[x] >= 2 * n - 1) return;
+    flag[pre[x]] = (x < n - 1 && (depth[x] & 1u) == 0) ? 1u : 0u;
+}
+
+// Three-pass exclusive scan of u32 (blocks of 1024 elements).
+__global__ __launch_bounds__(kB) void k_scan_blocks(const uint32_t* __restrict__ in, uint64_t n,
+                                                    uint32_t* __restrict__ out,
+                                                    uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t sh[4];
+    const uint64_t b0 = (uint64_t)blockIdx.x * 1024;
+    uint32_t v[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t i = b0 + 4 * threadIdx.x + k;
+        v[k] = i < n ? in[i] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_incl_scan(s, sh, &tot) - s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t i = b0 + 4 * threadIdx.x + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(kB) void k_scan_top(uint32_t* __restrict__ bsum, uint32_t nb,
+                                                 uint32_t* __restrict__ total) {
+    __shared__ uint32_t sh[4];
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < nb; b += kB) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < nb ? bsum[i] : 0u;
+        uint32_t tot;
+        const uint32_t inc = block_incl_scan(v, sh, &tot);
+        if (i < nb) bsum[i] = carry + inc - v;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+__global__ __launch_bounds__(kB) void k_scan_add(uint32_t* __restrict__ out, uint64_t n,
+                                                 const uint32_t* __restrict__ bsum) {
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < n) out[i] += bsum[i / 1024];
+}
+
+GPR_D void put_child(Bvh4Node& o, int slot, const float* b, int32_t ref) {
+    o.lox[slot] = b[0]; o.loy[slot] = b[1]; o.loz[slot] = b[2];
+    o.hix[slot] = b[3]; o.hiy[slot] = b[4]; o.hiz[slot] = b[5];
+    o.child[slot] = ref;
+}
+
+__global__ __launch_bounds__(kB) void k_bvh4(uint32_t n, const uint32_t* __restrict__ child,
+                                             const float* __restrict__ box,
+                                             const uint32_t* __restrict__ pre,
+                                             const uint32_t* __restrict__ depth,
+                                             const uint32_t* __restrict__ id4,  // by pre-order pos
+                                             Bvh4Node* __restrict__ out) {
+    const uint32_t x = blockIdx.x * kB + threadIdx.x;
+    if (x + 1 >= n || (depth[x] & 1u)) return;
+    const uint32_t nn = 2 * n - 1;
+    if (pre[x] >= nn || id4[pre[x]] >= n - 1) return;
+    Bvh4Node o;
+    int slot = 0;
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t c = child[2 * x + side];
+        if (c >= nn) return;  // malformed tree: leave the node unwritten
+        if (c >= n - 1) {
+            put_child(o, slot++, box + 6ull * c, ~(int32_t)(c - (n - 1)));
+        } else {
+            for (int s2 = 0; s2 < 2; ++s2) {
+                const uint32_t g = child[2 * c + s2];
+                if (g >= nn || pre[g] >= nn) return;
+                const int32_t ref = g >= n - 1 ? ~(int32_t)(g - (n - 1)) : (int32_t)id4[pre[g]];
+                put_child(o, slot++, box + 6ull * g, ref);
+            }
+        }
+    }
+    for (; slot < 4; ++slot) {
+        o.lox[slot] = o.loy[slot] = o.loz[slot] = INFINITY;
+        o.hix[slot] = o.hiy[slot] = o.hiz[slot] = INFINITY;
+        o.child[slot] = kEmptyChild;
+    }
+    o.pad[0] = x;
+    o.pad[1] = o.pad[2] = o.pad[3] = 0;
+    out[id4[pre[x]]] = o;
+}
+
+__global__ __launch_bounds__(kB) void k_tris(const float* __restrict__ pos, uint32_t n,
+                                             const uint32_t* __restrict__ sidx,
+                                             TriRec* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t t = sidx[i];
+    const float* p = pos + 9ull * t;
+    TriRec r;
+    r.a = make_float4(p[0], p[1], p[2], p[3]);
+    r.b = make_float4(p[4], p[5], p[6], p[7]);
+    r.c = make_float4(p[8], __uint_as_float(t), 0.f, 0.f);
+    out[i] = r;
+}
+
+struct Carve {
+    char* base;
+    size_t off;
+    template <class T>
+    T* take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = reinterpret_cast<T*>(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+struct SortLayout {
+    uint32_t* counts;
+    uint32_t* offs;
+    uint32_t* totals;
+    unsigned long long* tmp;
+    unsigned long long* alt;
+};
+size_t sort_layout(uint32_t n, char* base, SortLayout* L) {
+    const uint32_t ntiles = std::max(1u, div_up(n, kTileKeys));
+    Carve c{base, 0};
+    SortLayout l;
+    l.counts = c.take<uint32_t>((size_t)ntiles * kDigits);
+    l.offs = c.take<uint32_t>((size_t)ntiles * kDigits);
+    l.totals = c.take<uint32_t>(kDigits);
+    l.tmp = c.take<unsigned long long>(std::max(1u, n));
+    l.alt = c.take<unsigned long long>(std::max(1u, n));
+    if (L) *L = l;
+    return c.off + 256;
+}
+
+}  // namespace
+
+size_t sort_scratch_bytes(uint32_t n) { return sort_layout(n, nullptr, nullptr); }
+
+int sort_pairs(const uint32_t* keys, uint32_t* skeys, uint32_t* sidx, uint32_t n, void* scratch,
+               size_t scratch_bytes, hipStream_t st) {
+    if (n == 0) return GPR_OK;
+    if (scratch_bytes < sort_scratch_bytes(n)) return fail(GPR_ERR_INVALID, "sort scratch too small");
+    SortLayout L;
+    sort_layout(n, reinterpret_cast<char*>(scratch), &L);
+    const uint32_t ntiles = div_up(n, kTileKeys);
+    hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kB), 0, st, keys, n, L.counts);
+    hipLaunchKernelGGL(k_sort_scan, dim3(kDigits), dim3(kB), 0, st, L.counts, ntiles, L.offs, L.totals);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kB), 0, st, keys, n, L.offs, L.totals, L.tmp);
+    hipLaunchKernelGGL(k_sort_local, dim3(kDigits), dim3(kB), 0, st, L.totals, L.tmp, L.alt, skeys, sidx);
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+
+int build_bvh(gpr_scene* s) {
+    BvhDev& B = s->bvh;
+    const uint32_t n = (uint32_t)s->mat.size();
+    B.n = n;
+    if (n == 0) {
+        B.nbvh4 = 0;
+        return GPR_OK;
+    }
+    hipStream_t st = stream();
+    const uint32_t nn = 2 * n - 1;
+    const uint32_t nscan_blocks = div_up(nn, 1024);
+    const uint32_t npart = std::min(1024u, div_up(n, kB));
+    // ---- carve the arena
+    size_t need = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        Carve c{pass ? B.arena.as<char>() : nullptr, 0};
+        float* part = c.take<float>(6ull * npart);
+        float* bounds = c.take<float>(8);
+        uint32_t* keys = c.take<uint32_t>(n);
+        uint32_t* skeys = c.take<uint32_t>(n);
+        uint32_t* sidx = c.take<uint32_t>(n);
+        char* sortmem = c.take<char>(sort_scratch_bytes(n));
+        int32_t* w = c.take<int32_t>(n);
+        uint32_t* parent = c.take<uint32_t>(nn);
+        uint32_t* child = c.take<uint32_t>(2ull * n);
+        uint32_t* rlo = c.take<uint32_t>(n);
+        uint32_t* rhi = c.take<uint32_t>(n);
+        uint32_t* arrive = c.take<uint32_t>(n);
+        float* box = c.take<float>(6ull * nn);
+        uint32_t* root = c.take<uint32_t>(4);
+        uint32_t* pre = c.take<uint32_t>(nn);
+        uint32_t* depth = c.take<uint32_t>(nn);
+        uint32_t* flag = c.take<uint32_t>(nn);
+        uint32_t* id4 = c.take<uint32_t>(nn);
+        uint32_t* bsum = c.take<uint32_t>(nscan_blocks + 4);
+        uint32_t* total4 = c.take<uint32_t>(4);
+        gpr_ref_node* ref = c.take<gpr_ref_node>(nn);
+        Bvh4Node* bvh4 = c.take<Bvh4Node>(std::max(1u, n - 1));
+        TriRec* tris = c.take<TriRec>(n);
+        if (pass == 0) {
+            need = c.off + 256;
+            GPR_TRY(B.arena.alloc(need));
+            continue;
+        }
+        B.keys = keys;
+        B.skeys = skeys;
+        B.sidx = sidx;
+        B.ref = ref;
+        B.bvh4 = bvh4;
+        B.tris = tris;
+        const float* pos = s->d_pos.as<float>();
+        hipEvent_t ev[5];
+        for (int k = 0; k < 5; ++k) GPR_HIP(hipEventCreate(&ev[k]));
+        GPR_HIP(hipEventRecord(ev[0], st));
+        hipLaunchKernelGGL(k_centroid_partial, dim3(npart), dim3(kB), 0, st, pos, n, part);
+        hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(kB), 0, st, part, npart, bounds);
+        hipLaunchKernelGGL(k_morton, dim3(div_up(n, kB)), dim3(kB), 0, st, pos, n, bounds, keys);
+        GPR_HIP(hipEventRecord(ev[1], st));
+        GPR_TRY(sort_pairs(keys, skeys, sidx, n, sortmem, sort_scratch_bytes(n), st));
+        GPR_HIP(hipEventRecord(ev[2], st));
+        hipLaunchKernelGGL(k_fill_u32, dim3(std::min(1024u, div_up(n, kB))), dim3(kB), 0, st, arrive, 0u, (uint64_t)n);
+        hipLaunchKernelGGL(k_weights, dim3(div_up(n, kB)), dim3(kB), 0, st, skeys, n, w);
+        hipLaunchKernelGGL(k_tree, dim3(div_up(n, kB)), dim3(kB), 0, st, pos, n, sidx, w, parent,
+                           child, rlo, rhi, arrive, box, root);
+        GPR_HIP(hipEventRecord(ev[3], st));
+        hipLaunchKernelGGL(k_walkup, dim3(div_up(nn, kB)), dim3(kB), 0, st, n, parent, child, rlo, root, pre, depth);
+        hipLaunchKernelGGL(k_export_ref, dim3(div_up(nn, kB)), dim3(kB), 0, st, n, child, sidx, box, pre, ref);
+        hipLaunchKernelGGL(k_bvh4_flags, dim3(div_up(nn, kB)), dim3(kB), 0, st, n, pre, depth, flag);
+        hipLaunchKernelGGL(k_scan_blocks, dim3(nscan_blocks), dim3(kB), 0, st, flag, (uint64_t)nn, id4, bsum);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kB), 0, st, bsum, nscan_blocks, total4);
+        hipLaunchKernelGGL(k_scan_add, dim3(div_up(nn, kB)), dim3(kB), 0, st, id4, (uint64_t)nn, bsum);
+        if (n > 1)
+            hipLaunchKernelGGL(k_bvh4, dim3(div_up(n - 1, kB)), dim3(kB), 0, st, n, child, box, pre,
+                               depth, id4, bvh4);
+        hipLaunchKernelGGL(k_tris, dim3(div_up(n, kB)), dim3(kB), 0, st, pos, n, sidx, tris);
+        GPR_HIP(hipEventRecord(ev[4], st));
+        GPR_HIP(hipGetLastError());
+        uint32_t nb4 = 0;
+        GPR_HIP(hipMemcpyAsync(&nb4, total4, 4, hipMemcpyDeviceToHost, st));
+        GPR_HIP(hipStreamSynchronize(st));
+        B.nbvh4 = nb4;
+        if (n > 1 && (nb4 == 0 || nb4 > n - 1)) return fail(GPR_ERR_INVALID, "LBVH build produced an invalid BVH4 node count");
+        B.root4 = n == 1 ? ~(int32_t)0 : 0;
+        float ms;
+        for (int k = 0; k < 4; ++k) {
+            GPR_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            B.timing[k] = ms;
+        }
+        GPR_HIP(hipEventElapsedTime(&ms, ev[0], ev[4]));
+        B.timing[4] = ms;
+        for (int k = 0; k < 5; ++k) (void)hipEventDestroy(ev[k]);
+    }
+    return GPR_OK;
+}
+
+}  // namespace gpr

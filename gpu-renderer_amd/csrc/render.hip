@@ -1,0 +1,757 @@
+// render.hip -- wavefront path tracer on gfx950 (PathTracer::render,
+// src/renderer/pathtracer.cu:41-150, kernels of src/renderer/pathtracer_kernel.h).
+//
+// One sample of every pixel is a fixed launch sequence with no host round
+// trip:  gen -> extend(primary) -> max_depth x { shade -> shadow -> extend }
+// -> finish.  Queue sizes live on the device; each kernel reads the size it
+// consumes and early-exits, so the whole sample can be replayed as a hipGraph.
+//
+//   gen     primary rays (InitRandom/NextRandom jitter or PCG32), queue A
+//   extend  closest hit of a queue; for scattered rays also the BSDF-sampling
+//           MIS contribution (accumulate_BSDF_sampling_contribution) and, in
+//           reference RNG mode, its roulette (pathtracer_kernel.h:684-749)
+//   shade   shade_hit_with_MIS: emission, NEE shadow append, BSDF scatter
+//           append.  Appends are compacted in queue order: a wave ballot and
+//           mbcnt give lane offsets, LDS gives wave offsets, and a decoupled
+//           look-back over tiles (one 64-bit status word per tile, no other
+//           cross-XCD data) gives tile offsets -- so slot order equals the
+//           serial order of the reference's host loop.
+//   shadow  any-hit of the shadow queue + accumulate_light_sampling_contribution
+//
+// Per pixel the frame buffer receives emission, NEE and BSDF terms in the
+// same order as the reference kernels (one queue entry per pixel per kernel,
+// so the read-modify-write needs no atomics).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dev_bsdf.h"
+#include "traverse.h"
+
+namespace gpr {
+namespace {
+
+constexpr int kShadeBlock = 256;
+constexpr uint32_t kMaxBounce = 64;
+
+// Queue of path vertices / rays (SoA, capacity = pixel count).
+struct QueueView {
+    float4* o_tmin;   // origin, tmin
+    float4* d_tmax;   // direction, tmax
+    float4* hit;      // t, id bits, u, v
+    float4* w_pdf;    // throughput rgb, BSDF pdf
+    uint32_t* pix;    // pixel | specular << 31
+    unsigned long long* rng;  // PCG32 state (pcg mode)
+    uint32_t* seed;   // slot-indexed LCG seed (reference mode, persists across samples)
+};
+
+// Per-sample control block (zeroed by k_gen).
+struct Ctrl {
+    uint32_t sample;         // current sample index (advanced by k_finish)
+    uint32_t error;          // bit0: traversal stack overflow, bit1: look-back timeout
+    uint32_t shadow_size[kMaxBounce];
+    uint32_t scatter_size[kMaxBounce];
+    uint32_t tile_counter[kMaxBounce];
+};
+
+struct RenderArgs {
+    SceneView sv;
+    QueueView q[3];          // A, B (in/scatter alternate), S (shadow)
+    float4* fb;
+    Ctrl* ctrl;
+    unsigned long long* status;  // [kMaxBounce][ntiles] look-back words
+    int2* spill;
+    uint32_t npix;
+    uint32_t ntiles;
+    uint32_t rng_mode;
+    uint32_t rr_start;
+    unsigned long long seed;
+};
+
+GPR_D void fb_add(float4* fb, uint32_t pix, c3 v) {
+    float4 x = fb[pix];
+    x.x = x.x + v.r;
+    x.y = x.y + v.g;
+    x.z = x.z + v.b;
+    fb[pix] = x;
+}
+
+// Camera::generateRay (camera.h:86-119), Mitsuba raster mode.
+GPR_D void camera_ray(const SceneView& sv, float px, float py, v3& o, v3& d) {
+    const float* m = sv.r2c;
+    float x = m[0] * px + m[1] * py + m[2] * 0.f + m[3];
+    float y = m[4] * px + m[5] * py + m[6] * 0.f + m[7];
+    float z = m[8] * px + m[9] * py + m[10] * 0.f + m[11];
+    const float w = m[12] * px + m[13] * py + m[14] * 0.f + m[15];
+    v3 pc = V(x, y, z);
+    if (!(w == 1)) pc = pc / w;
+    const float* c = sv.c2w;
+    x = c[0] * 0.f + c[1] * 0.f + c[2] * 0.f + c[3];
+    y = c[4] * 0.f + c[5] * 0.f + c[6] * 0.f + c[7];
+    z = c[8] * 0.f + c[9] * 0.f + c[10] * 0.f + c[11];
+    const float ow = c[12] * 0.f + c[13] * 0.f + c[14] * 0.f + c[15];
+    o = V(x, y, z);
+    if (!(ow == 1)) o = o / ow;
+    const v3 n = norm3(pc);
+    d = norm3(V(c[0] * n.x + c[1] * n.y + c[2] * n.z, c[4] * n.x + c[5] * n.y + c[6] * n.z,
+                c[8] * n.x + c[9] * n.y + c[10] * n.z));
+}
+
+__global__ __launch_bounds__(256) void k_gen(RenderArgs a) {
+    const uint32_t tid = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t sample = a.ctrl->sample;
+    // clear the per-sample counters and look-back words
+    const uint32_t nctl = 3 * kMaxBounce;
+    for (uint32_t i = tid; i < nctl; i += gridDim.x * 256) a.ctrl->shadow_size[i] = 0u;
+    const uint64_t nstat = (uint64_t)kMaxBounce * a.ntiles;
+    for (uint64_t i = tid; i < nstat; i += (uint64_t)gridDim.x * 256) a.status[i] = 0ull;
+    if (tid >= a.npix) return;
+    const uint32_t W = a.sv.width;
+    const uint32_t x = tid % W, y = tid / W;
+    float jx, jy;
+    const QueueView& q = a.q[0];
+    if (a.rng_mode == GPR_RNG_REFERENCE) {
+        uint32_t s = tea16(tid, sample);  // InitRandom(idx, sample) (pathtracer_kernel.h:92)
+        jx = lcg_next(s);
+        jy = lcg_next(s);
+        q.seed[tid] = s;
+    } else {
+        const uint64_t inc = pcg_inc(sample, tid);
+        uint64_t st = pcg_init(a.seed, inc);
+        jx = pcg_unit(st, inc);
+        jy = pcg_unit(st, inc);
+        q.rng[tid] = st;
+    }
+    v3 o, d;
+    camera_ray(a.sv, (float)x + jx, (float)y + jy, o, d);
+    q.o_tmin[tid] = make_float4(o.x, o.y, o.z, 0.f);
+    q.d_tmax[tid] = make_float4(d.x, d.y, d.z, 1e34f);
+    q.w_pdf[tid] = make_float4(1.f, 1.f, 1.f, 0.f);
+    q.pix[tid] = tid;
+}
+
+GPR_D gpr_hit trace_one(const RenderArgs& a, const Ray3& r, LaneStack& st, bool any, bool ref_order) {
+    gpr_hit h;
+    if (ref_order) {
+        uint32_t nv = 0, nt = 0;
+        h = any ? ref_trace<true>(a.sv, r, &nv, &nt) : ref_trace<false>(a.sv, r, &nv, &nt);
+    } else {
+        const bool ok = any ? fast_trace<true>(a.sv, r, st, h) : fast_trace<false>(a.sv, r, st, h);
+        if (!ok) atomicOr(&a.ctrl->error, 1u);
+    }
+    return h;
+}
+
+// Light pdf (Triangle::pdf + pdf_lights, triangle.h:156-175, pathtracer_kernel.h:338-348)
+GPR_D float tri_area(const SceneView& sv, uint32_t t) {
+    const float* p = sv.pos + 9ull * t;
+    const v3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
+    return len3(cross3(p1 - p0, p2 - p0)) * 0.5f;
+}
+struct Geom {
+    v3 p, ns;
+};
+// Triangle::setupDifferential (triangle.h:104-138)
+GPR_D Geom setup_geom(const SceneView& sv, uint32_t t, float u, float v) {
+    const float* p = sv.pos + 9ull * t;
+    const v3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
+    Geom g;
+    g.p = p0 * (1 - u - v) + p1 * u + p2 * v;
+    if (sv.flags[t] & 1u) {
+        const float* n = sv.nrm + 9ull * t;
+        g.ns = norm3(load3(n) * (1 - u - v) + load3(n + 3) * u + load3(n + 6) * v);
+    } else {
+        g.ns = norm3(cross3(p0 - p2, p1 - p2));
+    }
+    return g;
+}
+GPR_D c3 emission_of(const SceneView& sv, uint32_t t) { return loadc(sv.mats[sv.mat[t]].emission); }
+GPR_D float power_heur(float a, float b) {
+    a *= a;
+    b *= b;
+    return a / (a + b);
+}
+
+// extend: closest hit; for scattered rays also the BSDF-sampling MIS term
+template <bool PRIMARY, bool REF_ORDER>
+__global__ __launch_bounds__(kTraceBlock) void k_extend(RenderArgs a, uint32_t bounce) {
+    __shared__ int2 lds[kStackLds * kTraceBlock];
+    LaneStack st;
+    st.lds = lds + threadIdx.x;
+    st.spill = a.spill + (uint64_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kStackGlobal;
+    st.sp = 0;
+    // queue: primary = A; at bounce b the scatter queue is q[(b+1)&1]
+    const QueueView& q = a.q[PRIMARY ? 0 : ((bounce + 1) & 1)];
+    const uint32_t n = PRIMARY ? a.npix : a.ctrl->scatter_size[bounce];
+    const uint32_t stride = gridDim.x * kTraceBlock;
+    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += stride) {
+        const float4 ot = q.o_tmin[i], dt = q.d_tmax[i];
+        Ray3 r;
+        r.o = V(ot.x, ot.y, ot.z);
+        r.d = V(dt.x, dt.y, dt.z);
+        r.tmin = ot.w;
+        r.tmax = dt.w;
+        gpr_hit h = trace_one(a, r, st, false, REF_ORDER);
+        if (!PRIMARY) {
+            const float4 wp = q.w_pdf[i];
+            const uint32_t pw = q.pix[i];
+            const uint32_t pix = pw & 0x7fffffffu;
+            const bool spec = (pw >> 31) != 0;
+            const c3 w = C(wp.x, wp.y, wp.z);
+            float u_rr = 0.f;
+            if (a.rng_mode == GPR_RNG_REFERENCE) u_rr = lcg_next(q.seed[i]);
+            if (h.id != -1 && !spec) {
+                const uint32_t t = (uint32_t)h.id;
+                const c3 le = emission_of(a.sv, t);
+                if (!black(le)) {
+                    const Geom lg = setup_geom(a.sv, t, h.u, h.v);
+                    float lpdf = 1.f / tri_area(a.sv, t);
+                    lpdf /= a.sv.nlights;
+                    const float d = h.t;
+                    lpdf *= d * d / fabsf(dot3(r.d, lg.ns));
+                    const c3 fl = dot3(-r.d, lg.ns) > 0 ? le : C(0.f);
+                    const float wgt = power_heur(wp.w, lpdf);
+                    fb_add(a.fb, pix, w * fl * wgt);
+                }
+            }
+            if (a.rng_mode == GPR_RNG_REFERENCE && cmax(w) < 1 && bounce > 3) {
+                // roulette without reweighting; a terminated path is a miss (pathtracer_kernel.h:738-748)
+                const float qq = fmax_h(0.05f, 1 - cmax(w));
+                if (u_rr < qq) h.id = -1;
+            }
+        }
+        q.hit[i] = make_float4(h.t, __int_as_float(h.id), h.u, h.v);
+    }
+}
+
+// shadow: any hit + accumulate_light_sampling_contribution (pathtracer_kernel.h:642-664)
+template <bool REF_ORDER>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(RenderArgs a, uint32_t bounce) {
+    __shared__ int2 lds[kStackLds * kTraceBlock];
+    LaneStack st;
+    st.lds = lds + threadIdx.x;
+    st.spill = a.spill + (uint64_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kStackGlobal;
+    st.sp = 0;
+    const QueueView& q = a.q[2];
+    const uint32_t n = a.ctrl->shadow_size[bounce];
+    const uint32_t stride = gridDim.x * kTraceBlock;
+    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += stride) {
+        const float4 ot = q.o_tmin[i], dt = q.d_tmax[i];
+        Ray3 r;
+        r.o = V(ot.x, ot.y, ot.z);
+        r.d = V(dt.x, dt.y, dt.z);
+        r.tmin = ot.w;
+        r.tmax = dt.w;
+        const gpr_hit h = trace_one(a, r, st, true, REF_ORDER);
+        if (h.id == -1) {
+            const float4 wp = q.w_pdf[i];
+            fb_add(a.fb, q.pix[i] & 0x7fffffffu, C(wp.x, wp.y, wp.z));
+        }
+    }
+}
+
+// ------------------------------------------------------------- shade ----
+struct Out {
+    v3 o, d;
+    float tmin, tmax;
+    c3 w;
+    float pdf;
+    bool spec;
+};
+
+// 64-bit look-back word: [63:62] flag (1 aggregate, 2 inclusive) [61:31] shadow [30:0] scatter
+GPR_D unsigned long long pack_status(uint32_t flag, uint32_t sh, uint32_t sc) {
+    return ((unsigned long long)flag << 62) | ((unsigned long long)sh << 31) | (unsigned long long)sc;
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bounce) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave[2][kShadeBlock / 64];
+    __shared__ uint32_t s_base[2];
+    const uint32_t n_in = bounce == 0 ? a.npix : a.ctrl->scatter_size[bounce - 1];
+    if (threadIdx.x == 0) s_tile = atomicAdd(&a.ctrl->tile_counter[bounce], 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t ntiles_in = (n_in + kShadeBlock - 1) / kShadeBlock;
+    if (tile >= ntiles_in) return;
+    const uint32_t i = tile * kShadeBlock + threadIdx.x;
+    const bool valid = i < n_in;
+    const QueueView& qi = a.q[bounce & 1];
+    const QueueView& qs = a.q[(bounce + 1) & 1];
+    const QueueView& qsh = a.q[2];
+    const SceneView& sv = a.sv;
+    bool has_sh = false, has_sc = false;
+    Out osh, osc;
+    uint32_t pix = 0;
+    unsigned long long rng_out = 0;
+    if (valid) {
+        const float4 ot = qi.o_tmin[i], dt = qi.d_tmax[i], hh = qi.hit[i], wp = qi.w_pdf[i];
+        const uint32_t pw = qi.pix[i];
+        pix = pw & 0x7fffffffu;
+        const bool spec = (pw >> 31) != 0;
+        const v3 rd = V(dt.x, dt.y, dt.z);
+        const c3 w = C(wp.x, wp.y, wp.z);
+        const int32_t hid = __float_as_int(hh.y);
+        float smp[8];
+        if (a.rng_mode == GPR_RNG_REFERENCE) {
+            uint32_t s = qi.seed[i];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) smp[k] = lcg_next(s);
+            qi.seed[i] = s;
+        } else {
+            const uint64_t inc = pcg_inc(a.ctrl->sample, pix);
+            uint64_t s = qi.rng[i];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) smp[k] = pcg_unit(s, inc);
+            rng_out = s;
+        }
+        if (hid != -1) {
+            const uint32_t t = (uint32_t)hid;
+            const Geom g = setup_geom(sv, t, hh.z, hh.w);
+            const v3 wo = -norm3(rd);
+            const gpr_material m = sv.mats[sv.mat[t]];
+            const Frame fr = make_frame(g.ns);
+            if (bounce == 0 || spec) {
+                const c3 le = loadc(m.emission);
+                if (!black(le)) {
+                    const c3 fl = dot3(wo, g.ns) > 0 ? le : C(0.f);
+                    fb_add(a.fb, pix, w * fl);
+                }
+            }
+            if (!bsdf_is_delta(m) && sv.nlights > 0) {
+                // sample_lights (pathtracer_kernel.h:323-336): uniform by count, uniform on the triangle
+                const uint32_t ln = sv.nlights;
+                uint32_t lid = (uint32_t)(smp[0] * ln);
+                if (lid > ln - 1) lid = ln - 1;
+                const uint32_t lt = sv.lights[lid];
+                const float sa = sqrtf(smp[1]);
+                const Geom lg = setup_geom(sv, lt, 1 - sa, smp[2] * sa);
+                float lpdf = 1.f / tri_area(sv, lt);
+                lpdf /= ln;
+                v3 wi = lg.p - g.p;
+                const float d2 = dot3(wi, wi);
+                const float dd = sqrtf(d2);
+                wi = wi / dd;
+                lpdf *= d2 / fabsf(dot3(wi, lg.ns));
+                if (isinf(lpdf)) lpdf = 0.f;
+                const c3 fl = dot3(-wi, lg.ns) > 0 ? emission_of(sv, lt) : C(0.f);
+                c3 fb;
+                float bpdf;
+                bsdf_eval_local(m, to_local(fr, wo), to_local(fr, wi), fb, bpdf);
+                const float wgt = power_heur(lpdf, bpdf);
+                if (lpdf > 0.f) {
+                    const c3 ow = w * fl * fb * wgt / lpdf;
+                    if (!black(ow)) {
+                        osh.o = g.p - rd * 1.0e-4f;
+                        osh.d = lg.p - osh.o;
+                        osh.tmin = 0.f;
+                        osh.tmax = 0.9999f;
+                        osh.w = ow;
+                        osh.pdf = 0.f;
+                        osh.spec = true;
+                        has_sh = true;
+                    }
+                }
+            }
+            {
+                const BSample bs = bsdf_sample(m, fr, wo, smp[5], smp[6], smp[4]);
+                c3 ow = bs.pdf == 0.f ? C(0.f) : w * bs.f / bs.pdf;
+                if (a.rng_mode != GPR_RNG_REFERENCE && bounce >= a.rr_start && !black(ow)) {
+                    const float qq = fmax_h(0.05f, 1 - cmax(ow));
+                    if (smp[7] < qq) ow = C(0.f);
+                    else ow = ow / (1 - qq);
+                }
+                if (!black(ow)) {
+                    osc.o = g.p;
+                    osc.d = bs.wi;
+                    osc.tmin = 1e-4f;
+                    osc.tmax = 1e38f;
+                    osc.w = ow;
+                    osc.pdf = bs.pdf;
+                    osc.spec = bs.specular;
+                    has_sc = true;
+                }
+            }
+        } else if (sv.has_env) {
+            fb_add(a.fb, pix, w * C(sv.env[0], sv.env[1], sv.env[2]));
+        }
+    }
+    // ---- order-preserving compaction of both appends
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long bsh = __ballot(has_sh), bsc = __ballot(has_sc);
+    const uint32_t lsh = __builtin_amdgcn_mbcnt_hi((uint32_t)(bsh >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bsh, 0u));
+    const uint32_t lsc = __builtin_amdgcn_mbcnt_hi((uint32_t)(bsc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bsc, 0u));
+    if (lane == 0) {
+        s_wave[0][wave] = (uint32_t)__popcll(bsh);
+        s_wave[1][wave] = (uint32_t)__popcll(bsc);
+    }
+    __syncthreads();
+    uint32_t wsh = 0, wsc = 0, tsh = 0, tsc = 0;
+#pragma unroll
+    for (int k = 0; k < kShadeBlock / 64; ++k) {
+        if (k < wave) {
+            wsh += s_wave[0][k];
+            wsc += s_wave[1][k];
+        }
+        tsh += s_wave[0][k];
+        tsc += s_wave[1][k];
+    }
+    if (threadIdx.x == 0) {
+        unsigned long long* st = a.status + (uint64_t)bounce * a.ntiles;
+        uint32_t psh = 0, psc = 0;
+        if (tile == 0) {
+            __hip_atomic_store(st, pack_status(2, tsh, tsc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(st + tile, pack_status(1, tsh, tsc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int j = (int)tile - 1;
+            uint32_t spins = 0;
+            while (j >= 0) {
+                const unsigned long long v = __hip_atomic_load(st + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t f = (uint32_t)(v >> 62);
+                if (f == 0) {
+                    // predecessor tiles took their ids earlier and are resident: this wait ends
+                    if (++spins > (1u << 26)) {
+                        atomicOr(&a.ctrl->error, 2u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                psh += (uint32_t)((v >> 31) & 0x7fffffffu);
+                psc += (uint32_t)(v & 0x7fffffffu);
+                if (f == 2) break;
+                --j;
+            }
+            __hip_atomic_store(st + tile, pack_status(2, psh + tsh, psc + tsc), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_base[0] = psh;
+        s_base[1] = psc;
+        if (tile == ntiles_in - 1) {
+            a.ctrl->shadow_size[bounce] = psh + tsh;
+            a.ctrl->scatter_size[bounce] = psc + tsc;
+        }
+    }
+    __syncthreads();
+    if (has_sh) {
+        const uint32_t s = s_base[0] + wsh + lsh;
+        qsh.o_tmin[s] = make_float4(osh.o.x, osh.o.y, osh.o.z, osh.tmin);
+        qsh.d_tmax[s] = make_float4(osh.d.x, osh.d.y, osh.d.z, osh.tmax);
+        qsh.w_pdf[s] = make_float4(osh.w.r, osh.w.g, osh.w.b, osh.pdf);
+        qsh.pix[s] = pix | 0x80000000u;
+    }
+    if (has_sc) {
+        const uint32_t s = s_base[1] + wsc + lsc;
+        qs.o_tmin[s] = make_float4(osc.o.x, osc.o.y, osc.o.z, osc.tmin);
+        qs.d_tmax[s] = make_float4(osc.d.x, osc.d.y, osc.d.z, osc.tmax);
+        qs.w_pdf[s] = make_float4(osc.w.r, osc.w.g, osc.w.b, osc.pdf);
+        qs.pix[s] = pix | (osc.spec ? 0x80000000u : 0u);
+        qs.rng[s] = rng_out;
+    }
+}
+
+__global__ void k_finish(Ctrl* c) {
+    if (threadIdx.x == 0) c->sample = c->sample + 1;
+}
+
+__global__ void k_clear_fb(float4* fb, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) fb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_set_sample(Ctrl* c, uint32_t s) {
+    if (threadIdx.x == 0) {
+        c->sample = s;
+        c->error = 0;
+    }
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+}  // namespace gpr
+
+using namespace gpr;
+
+struct gpr_renderer {
+    gpr_scene* scene = nullptr;
+    gpr_render_opts opts{};
+    uint32_t npix = 0, ntiles = 0;
+    uint32_t spp_done = 0;
+    DevBuf fb, ctrl, status, spill;
+    DevBuf qmem[3];
+    DevBuf seeds[2];
+    QueueView q[3];
+    int grid_trace = 0;
+    hipGraphExec_t graph = nullptr;
+    bool graph_ok = false;
+};
+
+namespace {
+int alloc_queue(DevBuf& mem, QueueView& q, uint32_t n, DevBuf* seed) {
+    const size_t per = 16 * 4 + 4 + 8;
+    GPR_TRY(mem.alloc((size_t)n * per + 8 * 256));
+    char* p = mem.as<char>();
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        char* r = p + off;
+        off += bytes;
+        return r;
+    };
+    q.o_tmin = reinterpret_cast<float4*>(take(16ull * n));
+    q.d_tmax = reinterpret_cast<float4*>(take(16ull * n));
+    q.hit = reinterpret_cast<float4*>(take(16ull * n));
+    q.w_pdf = reinterpret_cast<float4*>(take(16ull * n));
+    q.pix = reinterpret_cast<uint32_t*>(take(4ull * n));
+    q.rng = reinterpret_cast<unsigned long long*>(take(8ull * n));
+    q.seed = nullptr;
+    if (seed) {
+        GPR_TRY(seed->alloc(4ull * n));
+        GPR_HIP(hipMemsetAsync(seed->p, 0, 4ull * n, stream()));  // DEVICE_BUFFER allocations start zeroed (buffer.h:62)
+        q.seed = seed->as<uint32_t>();
+    }
+    return GPR_OK;
+}
+
+RenderArgs make_args(gpr_renderer* r) {
+    RenderArgs a{};
+    a.sv = r->scene->view();
+    for (int k = 0; k < 3; ++k) a.q[k] = r->q[k];
+    a.fb = r->fb.as<float4>();
+    a.ctrl = r->ctrl.as<Ctrl>();
+    a.status = r->status.as<unsigned long long>();
+    a.spill = r->spill.as<int2>();
+    a.npix = r->npix;
+    a.ntiles = r->ntiles;
+    a.rng_mode = r->opts.rng_mode;
+    a.rr_start = r->opts.rr_start;
+    a.seed = r->opts.seed;
+    return a;
+}
+
+int launch_sample(gpr_renderer* r, hipStream_t st) {
+    const RenderArgs a = make_args(r);
+    const bool ref = r->opts.traversal == GPR_TRAVERSAL_REF;
+    const dim3 tg(r->grid_trace), tb(kTraceBlock);
+    hipLaunchKernelGGL(k_gen, dim3(div_up(r->npix, 256)), dim3(256), 0, st, a);
+    if (ref) hipLaunchKernelGGL((k_extend<true, true>), tg, tb, 0, st, a, 0u);
+    else hipLaunchKernelGGL((k_extend<true, false>), tg, tb, 0, st, a, 0u);
+    for (uint32_t b = 0; b < r->opts.max_depth; ++b) {
+        hipLaunchKernelGGL(k_shade, dim3(r->ntiles), dim3(kShadeBlock), 0, st, a, b);
+        if (ref) {
+            hipLaunchKernelGGL((k_shadow<true>), tg, tb, 0, st, a, b);
+            hipLaunchKernelGGL((k_extend<false, true>), tg, tb, 0, st, a, b);
+        } else {
+            hipLaunchKernelGGL((k_shadow<false>), tg, tb, 0, st, a, b);
+            hipLaunchKernelGGL((k_extend<false, false>), tg, tb, 0, st, a, b);
+        }
+    }
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, a.ctrl);
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int gpr_renderer_create(gpr_scene* s, const gpr_render_opts* o, gpr_renderer** out) {
+    if (!s || !o || !out) return fail(GPR_ERR_INVALID, "gpr_renderer_create: null argument");
+    if (!s->committed) return fail(GPR_ERR_INVALID, "gpr_renderer_create: scene not committed");
+    if (o->max_depth > kMaxBounce) return fail(GPR_ERR_INVALID, "max_depth above 64");
+    if (o->rng_mode > GPR_RNG_PCG || o->traversal > GPR_TRAVERSAL_REF)
+        return fail(GPR_ERR_INVALID, "gpr_renderer_create: unknown mode");
+    const uint64_t npix = (uint64_t)s->width * s->height;
+    if (npix == 0 || npix >= (1ull << 31)) return fail(GPR_ERR_INVALID, "image size out of range");
+    gpr_renderer* r = new (std::nothrow) gpr_renderer();
+    if (!r) return fail(GPR_ERR_OOM, "gpr_renderer_create: host allocation failed");
+    r->scene = s;
+    r->opts = *o;
+    r->npix = (uint32_t)npix;
+    r->ntiles = div_up(npix, kShadeBlock);
+    GPR_HIP(hipSetDevice(s->device));
+    int rc = GPR_OK;
+    if (rc == GPR_OK) rc = r->fb.alloc(16ull * npix);
+    if (rc == GPR_OK) rc = r->ctrl.alloc(sizeof(Ctrl));
+    if (rc == GPR_OK) rc = r->status.alloc(8ull * kMaxBounce * r->ntiles);
+    for (int k = 0; k < 3 && rc == GPR_OK; ++k)
+        rc = alloc_queue(r->qmem[k], r->q[k], r->npix, k < 2 ? &r->seeds[k] : nullptr);
+    if (rc == GPR_OK) {
+        int cus = 0, per_cu = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend<false, false>, kTraceBlock, 0) != hipSuccess)
+            rc = fail(GPR_ERR_HIP, "occupancy query failed");
+        r->grid_trace = std::max(1, cus * std::max(1, per_cu));
+        r->grid_trace = std::min<int>(r->grid_trace, (int)div_up(npix, kTraceBlock));
+    }
+    if (rc == GPR_OK) rc = r->spill.alloc((size_t)r->grid_trace * kTraceBlock * kStackGlobal * sizeof(int2));
+    if (rc == GPR_OK) rc = gpr_renderer_reset(r);
+    if (rc != GPR_OK) {
+        delete r;
+        return rc;
+    }
+    *out = r;
+    return GPR_OK;
+}
+
+int gpr_renderer_reset(gpr_renderer* r) {
+    if (!r) return fail(GPR_ERR_INVALID, "gpr_renderer_reset: null renderer");
+    hipLaunchKernelGGL(k_clear_fb, dim3(div_up(r->npix, 256)), dim3(256), 0, stream(), r->fb.as<float4>(), r->npix);
+    hipLaunchKernelGGL(k_set_sample, dim3(1), dim3(64), 0, stream(), r->ctrl.as<Ctrl>(), 0u);
+    GPR_HIP(hipGetLastError());
+    r->spp_done = 0;
+    return GPR_OK;
+}
+
+int gpr_render(gpr_renderer* r, uint32_t spp) {
+    if (!r) return fail(GPR_ERR_INVALID, "gpr_render: null renderer");
+    if (r->scene->bvh.n == 0) return fail(GPR_ERR_UNSUPPORTED, "gpr_render: scene has no triangles");
+    hipStream_t st = stream();
+    for (uint32_t s = 0; s < spp; ++s) GPR_TRY(launch_sample(r, st));
+    r->spp_done += spp;
+    uint32_t err = 0;
+    GPR_HIP(hipMemcpyAsync(&err, reinterpret_cast<char*>(r->ctrl.p) + offsetof(Ctrl, error), 4,
+                           hipMemcpyDeviceToHost, st));
+    GPR_HIP(hipStreamSynchronize(st));
+    if (err) return fail(GPR_ERR_INVALID, "render error flags " + std::to_string(err));
+    return GPR_OK;
+}
+
+int gpr_renderer_read(gpr_renderer* r, float* rgb, uint32_t* cnt) {
+    if (!r) return fail(GPR_ERR_INVALID, "gpr_renderer_read: null renderer");
+    std::vector<float> tmp(4ull * r->npix);
+    GPR_HIP(hipMemcpyAsync(tmp.data(), r->fb.p, 16ull * r->npix, hipMemcpyDeviceToHost, stream()));
+    GPR_HIP(hipStreamSynchronize(stream()));
+    for (uint64_t i = 0; i < r->npix; ++i) {
+        if (rgb) {
+            rgb[3 * i + 0] = tmp[4 * i + 0];
+            rgb[3 * i + 1] = tmp[4 * i + 1];
+            rgb[3 * i + 2] = tmp[4 * i + 2];
+        }
+        if (cnt) cnt[i] = r->spp_done;
+    }
+    return GPR_OK;
+}
+
+int gpr_renderer_queue_sizes(gpr_renderer* r, uint32_t* shadow, uint32_t* scatter, uint32_t cap) {
+    if (!r) return fail(GPR_ERR_INVALID, "gpr_renderer_queue_sizes: null renderer");
+    Ctrl c;
+    GPR_HIP(hipMemcpyAsync(&c, r->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, stream()));
+    GPR_HIP(hipStreamSynchronize(stream()));
+    for (uint32_t b = 0; b < cap && b < kMaxBounce; ++b) {
+        if (shadow) shadow[b] = c.shadow_size[b];
+        if (scatter) scatter[b] = c.scatter_size[b];
+    }
+    return GPR_OK;
+}
+
+int gpr_renderer_free(gpr_renderer* r) {
+    if (r) {
+        (void)hipStreamSynchronize(stream());
+        delete r;
+    }
+    return GPR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- PNG ----
+namespace {
+uint32_t crc_table[256];
+bool crc_ready = false;
+uint32_t crc32(const unsigned char* p, size_t n, uint32_t c = 0xffffffffu) {
+    if (!crc_ready) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t v = i;
+            for (int k = 0; k < 8; ++k) v = (v & 1) ? 0xedb88320u ^ (v >> 1) : v >> 1;
+            crc_table[i] = v;
+        }
+        crc_ready = true;
+    }
+    for (size_t i = 0; i < n; ++i) c = crc_table[(c ^ p[i]) & 0xff] ^ (c >> 8);
+    return c;
+}
+void put32(std::vector<unsigned char>& o, uint32_t v) {
+    o.push_back((unsigned char)(v >> 24));
+    o.push_back((unsigned char)(v >> 16));
+    o.push_back((unsigned char)(v >> 8));
+    o.push_back((unsigned char)v);
+}
+void chunk(std::vector<unsigned char>& o, const char* type, const std::vector<unsigned char>& data) {
+    put32(o, (uint32_t)data.size());
+    std::vector<unsigned char> td(type, type + 4);
+    td.insert(td.end(), data.begin(), data.end());
+    o.insert(o.end(), td.begin(), td.end());
+    put32(o, crc32(td.data(), td.size()) ^ 0xffffffffu);
+}
+// 8-bit RGB PNG with stored (uncompressed) deflate blocks.
+int write_png(const std::string& path, uint32_t w, uint32_t h, const std::vector<unsigned char>& rgb) {
+    std::vector<unsigned char> raw;
+    raw.reserve((3ull * w + 1) * h);
+    for (uint32_t y = 0; y < h; ++y) {
+        raw.push_back(0);
+        raw.insert(raw.end(), rgb.begin() + 3ull * w * y, rgb.begin() + 3ull * w * (y + 1));
+    }
+    std::vector<unsigned char> z = {0x78, 0x01};
+    size_t pos = 0;
+    do {
+        const size_t len = std::min<size_t>(65535, raw.size() - pos);
+        z.push_back(pos + len == raw.size() ? 1 : 0);
+        z.push_back((unsigned char)(len & 0xff));
+        z.push_back((unsigned char)(len >> 8));
+        z.push_back((unsigned char)(~len & 0xff));
+        z.push_back((unsigned char)((~len >> 8) & 0xff));
+        z.insert(z.end(), raw.begin() + pos, raw.begin() + pos + len);
+        pos += len;
+    } while (pos < raw.size());
+    uint32_t s1 = 1, s2 = 0;
+    for (unsigned char c : raw) {
+        s1 = (s1 + c) % 65521u;
+        s2 = (s2 + s1) % 65521u;
+    }
+    put32(z, (s2 << 16) | s1);
+    std::vector<unsigned char> out = {0x89, 'P', 'N', 'G', 0x0d, 0x0a, 0x1a, 0x0a};
+    std::vector<unsigned char> ihdr;
+    put32(ihdr, w);
+    put32(ihdr, h);
+    ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});
+    chunk(out, "IHDR", ihdr);
+    chunk(out, "IDAT", z);
+    chunk(out, "IEND", {});
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return fail(GPR_ERR_IO, "cannot open " + path);
+    const size_t wr = std::fwrite(out.data(), 1, out.size(), f);
+    std::fclose(f);
+    if (wr != out.size()) return fail(GPR_ERR_IO, "short write to " + path);
+    return GPR_OK;
+}
+// GammaCorrect (spectrum.h:63-72)
+float gamma_correct(float v) {
+    if (v <= 0.0031308f) return 12.92f * v;
+    return 1.055f * std::pow(v, 1.f / 2.4f) - 0.055f;
+}
+}  // namespace
+
+extern "C" int gpr_renderer_write_png(gpr_renderer* r, const char* path) {
+    if (!r || !path) return fail(GPR_ERR_INVALID, "gpr_renderer_write_png: null argument");
+    const uint32_t w = r->scene->width, h = r->scene->height;
+    std::vector<float> rgb(3ull * r->npix);
+    GPR_TRY(gpr_renderer_read(r, rgb.data(), nullptr));
+    std::vector<unsigned char> px(3ull * r->npix);
+    const float n = (float)std::max(1u, r->spp_done);
+    // FrameBuffer::output (framebuffer.cpp:11-26): mean, gamma, x255 + 0.5,
+    // clamp, truncate; rows flipped on write (imageio.cpp:65).
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x) {
+            const uint64_t src = (uint64_t)y * w + x, dst = (uint64_t)(h - 1 - y) * w + x;
+            for (int c = 0; c < 3; ++c) {
+                float v = 255.f * gamma_correct(rgb[3 * src + c] / n) + 0.5f;
+                v = v < 0.f ? 0.f : (v > 255.f ? 255.f : v);
+                px[3 * dst + c] = (unsigned char)v;
+            }
+        }
+    return write_png(path, w, h, px);
+}

@@ -1,0 +1,240 @@
+// traverse.h -- closest-hit and any-hit traversal device code for gfx950.
+//
+// Two traversals share one triangle test:
+//   * ref_*  : the reference's visit order on its DFS layout (BVHLinearNode),
+//              its slab test and its per-thread 64-entry stack
+//              (src/renderer/pathtracer_kernel.h:154-308).  Used for exact
+//              reference semantics and for the traversal statistics.
+//   * fast_* : packed 128-B BVH4 nodes (one cache line per node), a robust slab
+//              test on all four children, nearest-first descent, leaf triangles
+//              tested as soon as their parent is visited, and a short per-lane
+//              stack in LDS that spills its oldest entries to a global overflow
+//              area.  Closest-hit results equal the reference's up to ties of t.
+#pragma once
+
+#include "dev_math.h"
+#include "gpr_internal.h"
+
+namespace gpr {
+
+constexpr int kStackLds = 16;      // LDS stack entries per lane (power of two)
+constexpr int kStackGlobal = 96;   // global overflow entries per lane
+constexpr int kTraceBlock = 256;
+
+struct Ray3 {
+    v3 o, d;
+    float tmin, tmax;
+};
+
+// triangle_intersect (pathtracer_kernel.h:119-152): float dots widened to
+// double, double 1/det; rejects det==0, u<0|u>1, v<0|u+v>1, t<tmin|t>tmax.
+GPR_D bool tri_hit_ref(v3 p0, v3 p1, v3 p2, v3 o, v3 d, float tmin, float tmax, float& th,
+                       float& uh, float& vh) {
+    const v3 e1 = p1 - p0;
+    const v3 e2 = p2 - p0;
+    const v3 P = cross3(d, e2);
+    const double det = (double)dot3(P, e1);
+    if (det == 0) return false;
+    const double inv_det = 1.0 / det;
+    const v3 T = o - p0;
+    const double u = (double)dot3(P, T) * inv_det;
+    if (u < 0.0 || u > 1.0) return false;
+    const v3 Q = cross3(T, e1);
+    const double v = (double)dot3(Q, d) * inv_det;
+    if (v < 0.0 || u + v > 1.0) return false;
+    const double t = (double)dot3(Q, e2) * inv_det;
+    if (t < (double)tmin || t > (double)tmax) return false;
+    th = (float)t;
+    uh = (float)u;
+    vh = (float)v;
+    return true;
+}
+
+// ------------------------------------------------- reference traversal ---
+// aabb_intersect (pathtracer_kernel.h:154-173)
+GPR_D bool ref_box(const gpr_ref_node& nd, v3 o, float tmin_r, float tmax_r, v3 inv, int nx, int ny,
+                   int nz) {
+    const float bx0 = nx ? nd.bmax[0] : nd.bmin[0], bx1 = nx ? nd.bmin[0] : nd.bmax[0];
+    const float by0 = ny ? nd.bmax[1] : nd.bmin[1], by1 = ny ? nd.bmin[1] : nd.bmax[1];
+    const float bz0 = nz ? nd.bmax[2] : nd.bmin[2], bz1 = nz ? nd.bmin[2] : nd.bmax[2];
+    float t0 = (bx0 - o.x) * inv.x;
+    float t1 = (bx1 - o.x) * inv.x;
+    const float ty0 = (by0 - o.y) * inv.y;
+    const float ty1 = (by1 - o.y) * inv.y;
+    if (t0 > ty1 || ty0 > t1) return false;
+    t0 = fmax_h(t0, ty0);
+    t1 = fmin_h(t1, ty1);
+    const float tz0 = (bz0 - o.z) * inv.z;
+    const float tz1 = (bz1 - o.z) * inv.z;
+    if (t0 > tz1 || tz0 > t1) return false;
+    t0 = fmax_h(t0, tz0);
+    t1 = fmin_h(t1, tz1);
+    return (t0 < tmax_r) && (t1 > tmin_r);
+}
+
+// trace_kernel / trace_shadow_kernel bodies (pathtracer_kernel.h:175-308).
+template <bool ANY>
+GPR_D gpr_hit ref_trace(const SceneView& sv, Ray3 r, uint32_t* nvisit, uint32_t* ntest) {
+    gpr_hit hit{INFINITY, -1, 0.f, 0.f};
+    const v3 inv = V(1 / r.d.x, 1 / r.d.y, 1 / r.d.z);
+    const int nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    int cur = 0, sp = 0;
+    int stack[64];
+    uint32_t visits = 0, tests = 0;
+    while (true) {
+        // every node is visited at most once; both checks only trip on a malformed layout
+        if ((uint32_t)cur >= sv.nref || visits > sv.nref) break;
+        const gpr_ref_node nd = sv.ref[cur];
+        ++visits;
+        if (ref_box(nd, r.o, r.tmin, r.tmax, inv, nx, ny, nz)) {
+            if (nd.leaf & 0xffu) {
+                if (nd.idx >= sv.ntri) break;
+                const float* p = sv.pos + 9ull * nd.idx;
+                float t, u, v;
+                ++tests;
+                if (tri_hit_ref(load3(p), load3(p + 3), load3(p + 6), r.o, r.d, r.tmin, r.tmax, t, u, v)) {
+                    r.tmax = t;
+                    hit = gpr_hit{t, (int32_t)nd.idx, u, v};
+                    if (ANY) break;
+                }
+                if (sp == 0) break;
+                cur = stack[--sp];
+            } else {
+                if (sp >= 64) break;  // unreachable for 30-bit keys; never write past the stack
+                stack[sp++] = cur + 1;
+                cur = (int)nd.idx;
+            }
+        } else {
+            if (sp == 0) break;
+            cur = stack[--sp];
+        }
+    }
+    *nvisit += visits;
+    *ntest += tests;
+    return hit;
+}
+
+// ------------------------------------------------------ BVH4 traversal ---
+struct LaneStack {
+    int2* lds;       // this lane's column: entry k at lds[k * kTraceBlock]
+    int2* spill;     // this lane's overflow area (kStackGlobal entries)
+    int sp;
+    GPR_D bool push(int2 e) {
+        if (sp >= kStackLds + kStackGlobal) return false;
+        const int slot = sp & (kStackLds - 1);
+        if (sp >= kStackLds) spill[sp - kStackLds] = lds[slot * kTraceBlock];
+        lds[slot * kTraceBlock] = e;
+        ++sp;
+        return true;
+    }
+    GPR_D int2 pop() {
+        --sp;
+        const int slot = sp & (kStackLds - 1);
+        const int2 e = lds[slot * kTraceBlock];
+        if (sp >= kStackLds) lds[slot * kTraceBlock] = spill[sp - kStackLds];
+        return e;
+    }
+};
+
+GPR_D void tri_rec_hit(const TriRec* tris, uint32_t ntri, int32_t leaf, const Ray3& r, float& tmax,
+                       gpr_hit& hit, bool& found) {
+    if ((uint32_t)leaf >= ntri) return;
+    const TriRec* t = tris + leaf;
+    const float4 a = t->a, b = t->b, c = t->c;
+    float th, uh, vh;
+    if (tri_hit_ref(V(a.x, a.y, a.z), V(a.w, b.x, b.y), V(b.z, b.w, c.x), r.o, r.d, r.tmin, tmax, th,
+                    uh, vh)) {
+        tmax = th;
+        hit = gpr_hit{th, (int32_t)__float_as_uint(c.y), uh, vh};
+        found = true;
+    }
+}
+
+// Closest hit (ANY=false) or any hit (ANY=true) against the BVH4.  Returns
+// false only if the stack capacity was exceeded (the ray is then reported
+// as a miss and the caller raises an error flag).
+template <bool ANY>
+GPR_D bool fast_trace(const SceneView& sv, const Ray3& r, LaneStack& st, gpr_hit& hit) {
+    hit = gpr_hit{INFINITY, -1, 0.f, 0.f};
+    float tmax = r.tmax;
+    bool found = false;
+    if (sv.root4 < 0) {  // single triangle
+        tri_rec_hit(sv.tris, sv.ntri, ~sv.root4, r, tmax, hit, found);
+        return true;
+    }
+    const v3 inv = V(1 / r.d.x, 1 / r.d.y, 1 / r.d.z);
+    const float kCons = 1.0f + 2.0f * 1.7881394e-07f;  // 1 + 2*gamma(3)
+    int32_t node = sv.root4;
+    st.sp = 0;
+    // a depth-first walk fetches every node at most once: the cap only trips on a malformed tree
+    for (uint32_t fetched = 0;; ++fetched) {
+        if ((uint32_t)node >= sv.nbvh4 || fetched > sv.nbvh4) return false;
+        const float4* np = reinterpret_cast<const float4*>(sv.bvh4 + node);
+        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+        const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+        const int32_t cha[4] = {ch.x, ch.y, ch.z, ch.w};
+        float tn[4];
+        bool hitc[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float x0 = (lxa[c] - r.o.x) * inv.x, x1 = (hxa[c] - r.o.x) * inv.x;
+            const float y0 = (lya[c] - r.o.y) * inv.y, y1 = (hya[c] - r.o.y) * inv.y;
+            const float z0 = (lza[c] - r.o.z) * inv.z, z1 = (hza[c] - r.o.z) * inv.z;
+            const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+            const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+            tn[c] = tnear;
+            hitc[c] = tnear <= tfar * kCons && cha[c] != kEmptyChild;
+        }
+        // leaf children first: they tighten tmax before the descent decision
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (hitc[c] && cha[c] < 0) {
+                tri_rec_hit(sv.tris, sv.ntri, ~cha[c], r, tmax, hit, found);
+                if (ANY && found) return true;
+            }
+        }
+        uint32_t key[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool go = hitc[c] && cha[c] >= 0 && tn[c] <= tmax * kCons;
+            key[c] = go ? ((__float_as_uint(fmaxf(tn[c], 0.f)) & ~3u) | (uint32_t)c) : 0xffffffffu;
+        }
+        // 4-key sorting network, ascending
+#define GPR_CSWAP(a, b)                   \
+    {                                     \
+        const uint32_t lo_ = min(key[a], key[b]); \
+        const uint32_t hi_ = max(key[a], key[b]); \
+        key[a] = lo_;                     \
+        key[b] = hi_;                     \
+    }
+        GPR_CSWAP(0, 1) GPR_CSWAP(2, 3) GPR_CSWAP(0, 2) GPR_CSWAP(1, 3) GPR_CSWAP(1, 2)
+#undef GPR_CSWAP
+        int cnt = (key[0] != 0xffffffffu) + (key[1] != 0xffffffffu) + (key[2] != 0xffffffffu) +
+                  (key[3] != 0xffffffffu);
+        if (cnt > 0) {
+            for (int k = cnt - 1; k >= 1; --k) {
+                const int c = key[k] & 3;
+                if (!st.push(make_int2(cha[c], __float_as_int(tn[c])))) return false;
+            }
+            node = cha[key[0] & 3];
+            continue;
+        }
+        // pop the nearest pending node that can still hold a closer hit
+        bool next = false;
+        while (st.sp > 0) {
+            const int2 e = st.pop();
+            if (ANY || __int_as_float(e.y) <= tmax * kCons) {
+                node = e.x;
+                next = true;
+                break;
+            }
+        }
+        if (!next) break;
+    }
+    return true;
+}
+
+}  // namespace gpr

@@ -1,0 +1,353 @@
+"""Python binding (ctypes) of the gpr C ABI declared in include/gpr.h.
+
+The native library gpr/libgpr.so (hand-written HIP for gfx950 + C++ host code)
+is the product; this module only marshals numpy arrays through the C ABI, the
+way a maintainer of the reference would bind it.  Importing fails loudly if the
+native library has not been built: there is no Python or CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpr.so")
+
+OK = 0
+ERR_INVALID, ERR_OOM, ERR_HIP, ERR_IO, ERR_PARSE, ERR_UNSUPPORTED, ERR_BUDGET = -1, -2, -3, -4, -5, -6, -7
+
+MAT_DIFFUSE, MAT_SPECULAR, MAT_ROUGH_CONDUCTOR, MAT_ROUGH_DIELECTRIC, MAT_ROUGH_PLASTIC = 0, 1, 2, 3, 4
+TRACE_ANY, TRACE_REF_ORDER, TRACE_DEVICE_PTRS = 1, 2, 4
+RNG_REFERENCE, RNG_PCG = 0, 1
+TRAVERSAL_FAST, TRAVERSAL_REF = 0, 1
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("tmin", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("id", "<i4"), ("u", "<f4"), ("v", "<f4")])
+NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("bmax", "<f4", 3), ("idx", "<u4"), ("leaf", "<u4")])
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("color", C.c_float * 3), ("color1", C.c_float * 3),
+                ("emission", C.c_float * 3), ("ior", C.c_float), ("alpha_x", C.c_float),
+                ("alpha_y", C.c_float), ("eta_i", C.c_float * 3), ("eta_t", C.c_float * 3),
+                ("k", C.c_float * 3)]
+
+
+MATERIAL_DTYPE = np.dtype([("type", "<u4"), ("color", "<f4", 3), ("color1", "<f4", 3),
+                           ("emission", "<f4", 3), ("ior", "<f4"), ("alpha_x", "<f4"),
+                           ("alpha_y", "<f4"), ("eta_i", "<f4", 3), ("eta_t", "<f4", 3),
+                           ("k", "<f4", 3)])
+assert MATERIAL_DTYPE.itemsize == C.sizeof(Material)
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("ntri", C.c_uint32), ("nlights", C.c_uint32), ("nmat", C.c_uint32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("fov_deg", C.c_float),
+                ("c2w", C.c_float * 16), ("has_env", C.c_uint32), ("env", C.c_float * 3),
+                ("bvh4_nodes", C.c_uint32), ("committed", C.c_uint32)]
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [("max_depth", C.c_uint32), ("rng_mode", C.c_uint32), ("rr_start", C.c_uint32),
+                ("seed", C.c_uint64), ("traversal", C.c_uint32)]
+
+
+class GprError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gpr error {code}: {msg}")
+        self.code = code
+
+
+def _load():
+    # One HIP runtime per process: when PyTorch is installed its bundled HIP /
+    # HSA runtime is loaded first, and libgpr's libamdhip64.so.7 dependency
+    # then binds to that same copy (matching soname) instead of a second one.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"native library {LIB_PATH} is not built; run __graft_entry__.build()")
+    return C.CDLL(LIB_PATH)
+
+
+lib = _load()
+_P = C.c_void_p
+_SIG = {
+    "gpr_last_error": (C.c_char_p, []),
+    "gpr_set_device": (C.c_int, [C.c_int]),
+    "gpr_set_stream": (C.c_int, [_P]),
+    "gpr_synchronize": (C.c_int, []),
+    "gpr_scene_create": (C.c_int, [C.POINTER(_P)]),
+    "gpr_scene_add_mesh": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _P,
+                                     C.c_uint32, C.POINTER(Material)]),
+    "gpr_scene_set_camera": (C.c_int, [_P, _P, C.c_float, C.c_uint32, C.c_uint32]),
+    "gpr_scene_set_camera_lookat": (C.c_int, [_P, _P, _P, _P, C.c_float, C.c_uint32, C.c_uint32]),
+    "gpr_scene_set_env": (C.c_int, [_P, _P]),
+    "gpr_scene_commit": (C.c_int, [_P]),
+    "gpr_scene_free": (C.c_int, [_P]),
+    "gpr_scene_get_info": (C.c_int, [_P, C.POINTER(SceneInfo)]),
+    "gpr_scene_export_flat": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "gpr_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "gpr_bvh_build": (C.c_int, [_P]),
+    "gpr_bvh_export_keys": (C.c_int, [_P, _P, _P, _P]),
+    "gpr_bvh_export_ref_layout": (C.c_int, [_P, _P, C.c_size_t]),
+    "gpr_bvh_last_timing": (C.c_int, [_P, _P]),
+    "gpr_sort_pairs_device": (C.c_int, [_P, _P, _P, C.c_uint32]),
+    "gpr_trace": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32]),
+    "gpr_trace_ref_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "gpr_renderer_create": (C.c_int, [_P, C.POINTER(RenderOpts), C.POINTER(_P)]),
+    "gpr_render": (C.c_int, [_P, C.c_uint32]),
+    "gpr_renderer_read": (C.c_int, [_P, _P, _P]),
+    "gpr_renderer_write_png": (C.c_int, [_P, C.c_char_p]),
+    "gpr_renderer_reset": (C.c_int, [_P]),
+    "gpr_renderer_queue_sizes": (C.c_int, [_P, _P, _P, C.c_uint32]),
+    "gpr_renderer_free": (C.c_int, [_P]),
+    "gpr_gen_soup": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_float, _P]),
+    "gpr_gen_rays": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, _P]),
+    "gpr_gen_shadow_rays": (C.c_int, [C.c_uint64, C.c_uint64, _P, C.c_uint64, _P]),
+    "gpr_scene_cornell": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+    "gpr_scene_sphere_grid": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+}
+for _name, (_res, _args) in _SIG.items():
+    if hasattr(lib, _name):
+        _f = getattr(lib, _name)
+        _f.restype = _res
+        _f.argtypes = _args
+
+
+def check(rc):
+    if rc != OK:
+        raise GprError(rc, lib.gpr_last_error().decode(errors="replace"))
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def material(type=MAT_DIFFUSE, color=(0.5, 0.5, 0.5), color1=(1, 1, 1), emission=(0, 0, 0),
+             ior=1.5, alpha=0.1, alpha_y=None, eta_i=(1, 1, 1), eta_t=(0.200438, 0.924033, 1.10221),
+             k=(3.91295, 2.45285, 2.14219)):
+    m = Material()
+    m.type = type
+    m.color[:] = color
+    m.color1[:] = color1
+    m.emission[:] = emission
+    m.ior = ior
+    m.alpha_x = alpha
+    m.alpha_y = alpha if alpha_y is None else alpha_y
+    m.eta_i[:] = eta_i
+    m.eta_t[:] = eta_t
+    m.k[:] = k
+    return m
+
+
+# ------------------------------------------------------------ generators --
+def gen_soup(seed, stream, n, sigma=0.005):
+    pos = np.empty((n, 9), np.float32)
+    check(lib.gpr_gen_soup(seed, stream, n, sigma, _ptr(pos)))
+    return pos
+
+
+def gen_rays(seed, stream, n):
+    rays = np.empty(n, RAY_DTYPE)
+    check(lib.gpr_gen_rays(seed, stream, n, _ptr(rays)))
+    return rays
+
+
+def gen_shadow_rays(seed, stream, rays):
+    out = np.empty(len(rays), RAY_DTYPE)
+    check(lib.gpr_gen_shadow_rays(seed, stream, _ptr(rays), len(rays), _ptr(out)))
+    return out
+
+
+# ----------------------------------------------------------------- scene --
+class Scene:
+    """Owns a gpr_scene handle."""
+
+    def __init__(self, handle=None):
+        if handle is None:
+            h = _P()
+            check(lib.gpr_scene_create(C.byref(h)))
+            handle = h
+        self.h = handle
+
+    @classmethod
+    def soup(cls, pos, mat=None):
+        """One mesh of unindexed triangles: pos is (ntri, 9) float32."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 9)
+        s = cls()
+        s.add_mesh(pos.reshape(-1, 3), corners=None, mat=mat)
+        return s
+
+    @classmethod
+    def cornell(cls, w=256, h=256):
+        hh = _P()
+        check(lib.gpr_scene_cornell(w, h, C.byref(hh)))
+        return cls(hh)
+
+    @classmethod
+    def sphere_grid(cls, w=1920, h=1080, subdiv=32):
+        hh = _P()
+        check(lib.gpr_scene_sphere_grid(w, h, subdiv, C.byref(hh)))
+        return cls(hh)
+
+    @classmethod
+    def load(cls, path):
+        hh = _P()
+        check(lib.gpr_scene_load(os.fsencode(path), C.byref(hh)))
+        return cls(hh)
+
+    def add_mesh(self, p, corners=None, n=None, uv=None, mat=None):
+        p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+        if corners is None:
+            ntri = len(p) // 3
+            corners = np.full((ntri, 3, 3), -1, np.int32)
+            corners[:, :, 0] = np.arange(3 * ntri, dtype=np.int32).reshape(ntri, 3)
+        corners = np.ascontiguousarray(corners, np.int32).reshape(-1, 3, 3)
+        n = None if n is None else np.ascontiguousarray(n, np.float32).reshape(-1, 3)
+        uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        m = mat if mat is not None else material()
+        check(lib.gpr_scene_add_mesh(self.h, _ptr(p), len(p), _ptr(n), 0 if n is None else len(n),
+                                     _ptr(uv), 0 if uv is None else len(uv), _ptr(corners),
+                                     len(corners), C.byref(m)))
+
+    def set_camera_lookat(self, target, origin, up, fov, w, h):
+        t = np.asarray(target, np.float32)
+        o = np.asarray(origin, np.float32)
+        u = np.asarray(up, np.float32)
+        check(lib.gpr_scene_set_camera_lookat(self.h, _ptr(t), _ptr(o), _ptr(u), fov, w, h))
+
+    def set_env(self, rgb):
+        a = np.asarray(rgb, np.float32)
+        check(lib.gpr_scene_set_env(self.h, _ptr(a)))
+
+    def commit(self):
+        check(lib.gpr_scene_commit(self.h))
+        return self
+
+    def info(self):
+        i = SceneInfo()
+        check(lib.gpr_scene_get_info(self.h, C.byref(i)))
+        return i
+
+    def flat(self):
+        i = self.info()
+        n = i.ntri
+        pos = np.empty((n, 9), np.float32)
+        nrm = np.empty((n, 9), np.float32)
+        uv = np.empty((n, 6), np.float32)
+        flags = np.empty(n, np.uint8)
+        mat = np.empty(n, np.uint32)
+        mats = np.empty(max(i.nmat, 1), MATERIAL_DTYPE)
+        check(lib.gpr_scene_export_flat(self.h, _ptr(pos), _ptr(nrm), _ptr(uv), _ptr(flags),
+                                        _ptr(mat), _ptr(mats)))
+        return dict(pos=pos, nrm=nrm, uv=uv, flags=flags, mat=mat, mats=mats[:i.nmat], info=i)
+
+    def build(self):
+        check(lib.gpr_bvh_build(self.h))
+
+    def keys(self):
+        n = self.info().ntri
+        k = np.empty(n, np.uint32)
+        sk = np.empty(n, np.uint32)
+        si = np.empty(n, np.uint32)
+        check(lib.gpr_bvh_export_keys(self.h, _ptr(k), _ptr(sk), _ptr(si)))
+        return k, sk, si
+
+    def ref_layout(self):
+        n = self.info().ntri
+        nodes = np.empty(max(2 * n - 1, 0), NODE_DTYPE)
+        if n:
+            check(lib.gpr_bvh_export_ref_layout(self.h, _ptr(nodes), len(nodes)))
+        return nodes
+
+    def timing(self):
+        t = np.zeros(5, np.float32)
+        check(lib.gpr_bvh_last_timing(self.h, _ptr(t)))
+        return t
+
+    def trace(self, rays, any_hit=False, ref_order=False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.empty(len(rays), HIT_DTYPE)
+        flags = (TRACE_ANY if any_hit else 0) | (TRACE_REF_ORDER if ref_order else 0)
+        check(lib.gpr_trace(self.h, _ptr(rays), _ptr(hits), len(rays), flags))
+        return hits
+
+    def trace_device(self, rays_ptr, hits_ptr, n, any_hit=False, ref_order=False):
+        flags = TRACE_DEVICE_PTRS | (TRACE_ANY if any_hit else 0) | (TRACE_REF_ORDER if ref_order else 0)
+        check(lib.gpr_trace(self.h, C.c_void_p(rays_ptr), C.c_void_p(hits_ptr), n, flags))
+
+    def ref_stats(self):
+        nv, nt = C.c_uint64(), C.c_uint64()
+        check(lib.gpr_trace_ref_stats(self.h, C.byref(nv), C.byref(nt)))
+        return nv.value, nt.value
+
+    def free(self):
+        if self.h:
+            lib.gpr_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Renderer:
+    def __init__(self, scene, max_depth=5, rng_mode=RNG_PCG, rr_start=3, seed=1,
+                 traversal=TRAVERSAL_FAST):
+        self.scene = scene
+        o = RenderOpts(max_depth, rng_mode, rr_start, seed, traversal)
+        h = _P()
+        check(lib.gpr_renderer_create(scene.h, C.byref(o), C.byref(h)))
+        self.h = h
+        i = scene.info()
+        self.w, self.h_px = i.width, i.height
+
+    def render(self, spp):
+        check(lib.gpr_render(self.h, spp))
+
+    def read(self):
+        n = self.w * self.h_px
+        rgb = np.empty((n, 3), np.float32)
+        cnt = np.empty(n, np.uint32)
+        check(lib.gpr_renderer_read(self.h, _ptr(rgb), _ptr(cnt)))
+        return rgb, cnt
+
+    def write_png(self, path):
+        check(lib.gpr_renderer_write_png(self.h, os.fsencode(path)))
+
+    def reset(self):
+        check(lib.gpr_renderer_reset(self.h))
+
+    def queue_sizes(self, cap=64):
+        sh = np.zeros(cap, np.uint32)
+        sc = np.zeros(cap, np.uint32)
+        check(lib.gpr_renderer_queue_sizes(self.h, _ptr(sh), _ptr(sc), cap))
+        return sh, sc
+
+    def free(self):
+        if self.h:
+            lib.gpr_renderer_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def sort_pairs_device(keys_ptr, skeys_ptr, sidx_ptr, n):
+    check(lib.gpr_sort_pairs_device(C.c_void_p(keys_ptr), C.c_void_p(skeys_ptr), C.c_void_p(sidx_ptr), n))
+
+
+def set_stream(ptr):
+    check(lib.gpr_set_stream(None if not ptr else C.c_void_p(ptr)))
+
+
+def synchronize():
+    check(lib.gpr_synchronize())

@@ -1,0 +1,212 @@
+/*
+ * gpr.h -- C ABI of the MI355X-native wavefront path-tracing core.
+ *
+ * Drop-in boundary for the path that slongle/GPU-Renderer exposes through
+ * in-process C++ types.  Each entry point names the reference interface it
+ * replaces (file:line in the reference tree).  Conventions:
+ *   - every call returns an int status: GPR_OK (0) or a negative GPR_ERR_*;
+ *     gpr_last_error() gives the message of the last failing call on this
+ *     thread (the reference prints and exit(1)s: src/utility/buffer.h:3-12);
+ *   - handles are opaque; device memory is owned by the library; host
+ *     buffers are owned by the caller;
+ *   - one handle is used from one host thread at a time;
+ *   - work is issued on the stream set with gpr_set_stream (default: the
+ *     null stream) and every call that returns host data synchronizes it.
+ */
+#ifndef GPR_H
+#define GPR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPR_OK 0
+#define GPR_ERR_INVALID -1     /* bad argument / state */
+#define GPR_ERR_OOM -2         /* device or host allocation failed */
+#define GPR_ERR_HIP -3         /* HIP runtime error */
+#define GPR_ERR_IO -4          /* file could not be read / written */
+#define GPR_ERR_PARSE -5       /* scene file malformed */
+#define GPR_ERR_UNSUPPORTED -6 /* feature outside the supported subset */
+#define GPR_ERR_BUDGET -7      /* request exceeds the device-memory budget */
+
+/* Ray: origin, direction, [tmin, tmax] -- Ray (src/renderer/ray.h:5-11), 32 B. */
+typedef struct {
+    float o[3];
+    float d[3];
+    float tmin;
+    float tmax;
+} gpr_ray;
+
+/* Hit: t, triangle id (-1 = miss), barycentrics -- Hit (src/renderer/ray.h:13-19), 16 B.
+ * Misses are returned as {t=+inf, id=-1, u=0, v=0}. */
+typedef struct {
+    float t;
+    int32_t id;
+    float u;
+    float v;
+} gpr_hit;
+
+/* Reference BVH node layout -- BVHLinearNode (src/renderer/bvh.h:34-43), 32 B:
+ * AABB, union{triangle index (leaf), right child index (interior)}, leaf flag
+ * in the low byte of `leaf` (upper bytes zero).  Left child = own index + 1. */
+typedef struct {
+    float bmin[3];
+    float bmax[3];
+    uint32_t idx;
+    uint32_t leaf;
+} gpr_ref_node;
+
+/* Material with constant textures -- Material (src/renderer/material.h:6-35),
+ * factories in src/renderer/loaders/mitsubaloader.cpp:289-441. */
+#define GPR_MAT_DIFFUSE 0          /* Lambert */
+#define GPR_MAT_SPECULAR 1         /* smooth dielectric / mirror (FresnelSpecular) */
+#define GPR_MAT_ROUGH_CONDUCTOR 2  /* GGX reflection, conductor Fresnel */
+#define GPR_MAT_ROUGH_DIELECTRIC 3 /* GGX reflection + transmission (Walter 2007) */
+#define GPR_MAT_ROUGH_PLASTIC 4    /* Lambert + GGX, dielectric Fresnel */
+typedef struct {
+    uint32_t type;
+    float color[3];    /* reflectance / specularReflectance / diffuseReflectance */
+    float color1[3];   /* plastic: specularReflectance */
+    float emission[3]; /* area-light radiance (0 = not a light) */
+    float ior;
+    float alpha_x, alpha_y;
+    float eta_i[3], eta_t[3], k[3];
+} gpr_material;
+
+typedef struct gpr_scene gpr_scene;
+typedef struct gpr_renderer gpr_renderer;
+
+/* ------------------------------------------------------------ runtime --- */
+const char* gpr_last_error(void);
+/* Select the HIP device used by handles created afterwards (default 0). */
+int gpr_set_device(int device);
+/* Stream (hipStream_t) for all subsequent work; NULL = null stream. */
+int gpr_set_stream(void* hip_stream);
+int gpr_synchronize(void);
+
+/* -------------------------------------------------------------- scene --- */
+/* Scene(filename) (src/renderer/scene.cpp:4-28) is split into create / add /
+ * commit so that procedural and loaded scenes share one path. */
+int gpr_scene_create(gpr_scene** out);
+/* One TriangleMesh (src/renderer/triangle.h:28-49): positions, optional
+ * normals and uvs, corner table of (v, vt, vn) int triples (-1 = absent),
+ * per-mesh material.  Triangles are appended in mesh order, then face
+ * order, which defines the triangle id (scene.cpp:7-14). */
+int gpr_scene_add_mesh(gpr_scene* s, const float* p, uint32_t np, const float* n, uint32_t nn,
+                       const float* uv, uint32_t nuv, const int32_t* corners, uint32_t ntri,
+                       const gpr_material* mat);
+/* Camera::setup (src/renderer/camera.h:21-37), Mitsuba raster mode. c2w is
+ * the row-major cameraToWorld matrix. */
+int gpr_scene_set_camera(gpr_scene* s, const float c2w[16], float fov_deg, uint32_t width,
+                         uint32_t height);
+/* LookAt (src/renderer/transform.cpp:220-248) + Camera::setup. */
+int gpr_scene_set_camera_lookat(gpr_scene* s, const float target[3], const float origin[3],
+                                const float up[3], float fov_deg, uint32_t width, uint32_t height);
+/* Constant environment light (ParseRecord::createLight "constant",
+ * mitsubaloader.cpp:467-471). */
+int gpr_scene_set_env(gpr_scene* s, const float rgb[3]);
+/* Flatten meshes, upload, build the light list (scene.cpp:16-20) and the
+ * BVH on the device (replaces LBVH_build, src/renderer/bvh.h:186-221). */
+int gpr_scene_commit(gpr_scene* s);
+int gpr_scene_free(gpr_scene* s);
+
+typedef struct {
+    uint32_t ntri;
+    uint32_t nlights;
+    uint32_t nmat;
+    uint32_t width, height;
+    float fov_deg;
+    float c2w[16];
+    uint32_t has_env;
+    float env[3];
+    uint32_t bvh4_nodes; /* packed 128-B traversal nodes */
+    uint32_t committed;
+} gpr_scene_info;
+int gpr_scene_get_info(const gpr_scene* s, gpr_scene_info* out);
+/* Flattened per-triangle data in triangle-id order (what the kernels read):
+ * pos ntri*9, nrm ntri*9, uv ntri*6, flags ntri (bit0 normals, bit1 uvs),
+ * mat ntri; mats nmat.  Any pointer may be NULL. */
+int gpr_scene_export_flat(const gpr_scene* s, float* pos, float* nrm, float* uv, uint8_t* flags,
+                          uint32_t* mat, gpr_material* mats);
+
+/* Mitsuba-XML subset / .obj / .gprs scene file -> committed scene
+ * (load_scene, src/renderer/loaders/loader.h:11-29). */
+int gpr_scene_load(const char* path, gpr_scene** out);
+
+/* ---------------------------------------------------------------- BVH --- */
+/* Rebuild the BVH of a committed scene on the device (LBVH_build,
+ * bvh.h:186-221): centroid bounds, 30-bit Morton keys, radix sort,
+ * radix-tree split with the equal-key midpoint rule, bottom-up refit,
+ * reference DFS layout and packed 128-B BVH4 nodes. */
+int gpr_bvh_build(gpr_scene* s);
+/* Morton keys per triangle id, and the sorted (key, triangle id) pairs
+ * (bvh.h:199-208; ties in triangle-id order). Any pointer may be NULL. */
+int gpr_bvh_export_keys(const gpr_scene* s, uint32_t* keys, uint32_t* sorted_keys,
+                        uint32_t* sorted_idx);
+/* Reference DFS layout (bvh.h:121-141, 2*ntri-1 nodes). */
+int gpr_bvh_export_ref_layout(const gpr_scene* s, gpr_ref_node* out, size_t cap);
+/* Device time (ms) of each stage of the last build:
+ * [0] bounds+morton, [1] sort, [2] tree+refit, [3] layouts, [4] total. */
+int gpr_bvh_last_timing(const gpr_scene* s, float ms[5]);
+
+/* Standalone LDS-staged radix sort of n 30-bit keys into (key, index) pairs,
+ * stable (ties in index order).  Device pointers. */
+int gpr_sort_pairs_device(const uint32_t* keys, uint32_t* sorted_keys, uint32_t* sorted_idx,
+                          uint32_t n);
+
+/* -------------------------------------------------------------- trace --- */
+#define GPR_TRACE_ANY 1u         /* any-hit: stop at the first accepted hit */
+#define GPR_TRACE_REF_ORDER 2u   /* reference visit order on the DFS layout */
+#define GPR_TRACE_DEVICE_PTRS 4u /* rays / hits are device pointers */
+/* trace / trace_shadow (src/renderer/pathtracer_kernel.h:235-246, 310-321). */
+int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags);
+/* Node visits and triangle tests of the last GPR_TRACE_REF_ORDER call. */
+int gpr_trace_ref_stats(const gpr_scene* s, uint64_t* node_visits, uint64_t* tri_tests);
+
+/* ----------------------------------------------------------- renderer --- */
+#define GPR_RNG_REFERENCE 0u /* TEA seed + LCG, slot-indexed seeds (pathtracer_kernel.h:92, rayqueue.h) */
+#define GPR_RNG_PCG 1u       /* PCG32 state carried with the path, unbiased roulette */
+#define GPR_TRAVERSAL_FAST 0u
+#define GPR_TRAVERSAL_REF 1u
+typedef struct {
+    uint32_t max_depth; /* PTOptions::m_max_path_length (pathtracer.h:7-14) */
+    uint32_t rng_mode;  /* GPR_RNG_* */
+    uint32_t rr_start;  /* pcg: roulette when bounce >= rr_start */
+    uint64_t seed;      /* pcg seed */
+    uint32_t traversal; /* GPR_TRAVERSAL_* */
+} gpr_render_opts;
+/* PathTracer(filename) + init() (pathtracer.cu:8-39). */
+int gpr_renderer_create(gpr_scene* s, const gpr_render_opts* opts, gpr_renderer** out);
+/* spp x PathTracer::render() (pathtracer.cu:41-150). */
+int gpr_render(gpr_renderer* r, uint32_t spp);
+/* Linear radiance sum and per-pixel sample count (FrameBuffer, framebuffer.h:33-70). */
+int gpr_renderer_read(gpr_renderer* r, float* rgb_sum, uint32_t* sample_num);
+/* FrameBuffer::output (framebuffer.cpp:11-26): mean, sRGB gamma, 8-bit PNG. */
+int gpr_renderer_write_png(gpr_renderer* r, const char* path);
+/* PathTracer::reset (pathtracer.cu:195-200). */
+int gpr_renderer_reset(gpr_renderer* r);
+/* Queue sizes of the last sample: in/shadow/scatter per bounce (up to cap
+ * bounces); the reference's m_sum_bounce (pathtracer.cu:96). */
+int gpr_renderer_queue_sizes(gpr_renderer* r, uint32_t* shadow, uint32_t* scatter, uint32_t cap);
+int gpr_renderer_free(gpr_renderer* r);
+
+/* --------------------------------------------------- synthetic inputs --- */
+/* Seeded generators of the benchmark workloads (PCG32 streams). */
+int gpr_gen_soup(uint64_t seed, uint64_t stream, uint32_t n, float sigma, float* pos);
+int gpr_gen_rays(uint64_t seed, uint64_t stream, uint64_t n, gpr_ray* rays);
+/* Shadow rays from rays[i].o to a uniform in-cube target, t in [0, 0.9999]. */
+int gpr_gen_shadow_rays(uint64_t seed, uint64_t stream, const gpr_ray* rays, uint64_t n,
+                        gpr_ray* out);
+/* Procedural Cornell box (36 triangles, Lambert, one 0.5x0.5 light of radiance 17). */
+int gpr_scene_cornell(uint32_t width, uint32_t height, gpr_scene** out);
+/* 8x8 UV spheres (64x32 segments) on a ground plane, two quad lights, mixed
+ * GGX conductor / dielectric / Lambert materials. */
+int gpr_scene_sphere_grid(uint32_t width, uint32_t height, uint32_t subdiv, gpr_scene** out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

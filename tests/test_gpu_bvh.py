@@ -1,0 +1,83 @@
+"""GPU LBVH build vs the oracle: Morton keys, sorted (key, id) pairs and the
+reference DFS layout must be bit-exact (bvh.h:186-221)."""
+import numpy as np
+import pytest
+
+import gpr
+import gpu_scenes as G
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "n1": lambda: G.soup(1),
+    "n2": lambda: G.soup(2),
+    "n3": lambda: G.soup(3),
+    "soup4096": lambda: G.soup(4096),
+    "soup_odd": lambda: G.soup(77777, seed=5),
+    "dups": lambda: G.duplicate_heavy(20000),
+    "identical": lambda: G.identical(1000),
+    "identical_big": lambda: G.identical(9000),  # one bucket larger than the LDS tile
+}
+
+
+def check_build(pos, scene):
+    n = len(pos)
+    keys, sk, si = scene.keys()
+    okeys = O.morton_keys(pos)
+    print(f"[bvh] n={n} keys", flush=True)
+    assert np.array_equal(keys, okeys)
+    osk, osi = O.sort_pairs(okeys)
+    assert np.array_equal(sk, osk)
+    assert np.array_equal(si, osi)
+    nodes = scene.ref_layout()
+    onodes, _, _ = O.lbvh(pos)
+    for f in ("bmin", "bmax", "idx", "leaf"):
+        assert np.array_equal(nodes[f], onodes[f]), f
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_build_matches_oracle(name):
+    pos = CASES[name]()
+    s = G.scene_from_soup(pos)
+    check_build(pos, s)
+    s.build()  # a rebuild must reproduce the same arrays
+    check_build(pos, s)
+
+
+def test_build_cornell():
+    s = gpr.Scene.cornell(64, 64)
+    f = s.flat()
+    check_build(f["pos"], s)
+
+
+def test_build_1m_soup():
+    pos = G.soup(1 << 20)
+    s = G.scene_from_soup(pos)
+    check_build(pos, s)
+    t = s.timing()
+    print("[bvh] 1M build ms (morton, sort, tree, layouts, total):", t.tolist(), flush=True)
+
+
+def test_sort_device_entry_point():
+    import torch
+    rng = np.random.default_rng(4)
+    for n in (1, 5, 4096, 4097, 300001):
+        keys = rng.integers(0, 1 << 30, n).astype(np.uint32)
+        keys[: n // 3] = keys[0]  # a long run of equal keys
+        dk = torch.from_numpy(keys.view(np.int32)).cuda()
+        sk = torch.empty_like(dk)
+        si = torch.empty_like(dk)
+        gpr.sort_pairs_device(dk.data_ptr(), sk.data_ptr(), si.data_ptr(), n)
+        torch.cuda.synchronize()
+        osk, osi = O.sort_pairs(keys)
+        assert np.array_equal(sk.cpu().numpy().view(np.uint32), osk)
+        assert np.array_equal(si.cpu().numpy().view(np.uint32), osi)
+
+
+def test_empty_scene():
+    s = gpr.Scene().commit()
+    assert s.info().ntri == 0
+    rays = gpr.gen_rays(1, 1, 100)
+    h = s.trace(rays)
+    assert np.all(h["id"] == -1)
