@@ -1,0 +1,106 @@
+"""Wavefront integrator on the GPU vs the oracle's restatement of
+PathTracer::render (pathtracer.cu:41-150, pathtracer_kernel.h).
+
+With the reference visit order on the GPU the float work is the same, so the
+frame buffers must agree bit for bit in both RNG modes (reference TEA/LCG with
+slot-indexed seeds, and PCG32 carried with the path).  With the fast BVH4
+traversal only ties of t may differ: per-pixel relative error <= 1e-4 on
+every pixel but a tiny tail, which must stay within a loose bound.
+"""
+import numpy as np
+import pytest
+
+import gpr
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_render(scene, spp, depth, rng_mode, seed, rr_start=3):
+    f = scene.flat()
+    info = f["info"]
+    osc = O.make_scene(f, np.array(info.c2w[:], np.float32), info.fov_deg, info.width, info.height,
+                       env=(list(info.env) if info.has_env else None))
+    nodes, _, _ = O.lbvh(f["pos"])
+    rgb, cnt = O.render(osc, nodes, spp, depth, rng_mode, seed=seed, rr_start=rr_start)
+    return rgb, cnt
+
+
+def gpu_render(scene, spp, depth, rng_mode, seed, traversal, rr_start=3):
+    r = gpr.Renderer(scene, max_depth=depth, rng_mode=rng_mode, rr_start=rr_start, seed=seed,
+                     traversal=traversal)
+    r.render(spp)
+    rgb, cnt = r.read()
+    sh, sc = r.queue_sizes(depth)
+    r.free()
+    return rgb, cnt, sh, sc
+
+
+@pytest.mark.parametrize("rng_mode", [gpr.RNG_REFERENCE, gpr.RNG_PCG])
+def test_cornell_bit_exact_reference_order(rng_mode):
+    s = gpr.Scene.cornell(48, 40)
+    print(f"[render] cornell 48x40 rng={rng_mode}", flush=True)
+    g, cnt, sh, sc = gpu_render(s, 4, 6, rng_mode, 1, gpr.TRAVERSAL_REF)
+    o, ocnt = oracle_render(s, 4, 6, rng_mode, 1)
+    assert np.array_equal(cnt, ocnt)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    assert np.all(sc <= 48 * 40) and g.sum() > 0
+
+
+def test_sphere_grid_bit_exact_reference_order():
+    s = gpr.Scene.sphere_grid(64, 36, 8)
+    print("[render] sphere grid 64x36 subdiv 8", flush=True)
+    g, _, _, _ = gpu_render(s, 2, 8, gpr.RNG_PCG, 7, gpr.TRAVERSAL_REF)
+    o, _ = oracle_render(s, 2, 8, gpr.RNG_PCG, 7)
+    bad = np.nonzero(np.any(g.view(np.uint32) != o.view(np.uint32), axis=1))[0]
+    assert len(bad) == 0, (len(bad), bad[:10], g[bad[:3]], o[bad[:3]])
+
+
+def rel_close(g, o, tol=1e-4):
+    den = np.maximum(np.abs(o), 1e-6)
+    return np.all(np.abs(g - o) <= tol * den + 1e-7, axis=1)
+
+
+def test_cornell_c1_fast_traversal_pcg():
+    # the C1 configuration: 256x256, 16 spp, depth 5, PCG32 seed 1
+    s = gpr.Scene.cornell(256, 256)
+    g, _, _, _ = gpu_render(s, 16, 5, gpr.RNG_PCG, 1, gpr.TRAVERSAL_FAST)
+    o, _ = oracle_render(s, 16, 5, gpr.RNG_PCG, 1)
+    ok = rel_close(g, o)
+    frac = ok.mean()
+    print(f"[render] C1 per-pixel 1e-4 agreement {frac:.6f} ({(~ok).sum()} pixels differ)", flush=True)
+    assert frac >= 0.999
+    # the mean image agrees far more tightly than a single path flip
+    assert abs(g.mean() - o.mean()) <= 1e-3 * o.mean()
+
+
+def test_cornell_fast_traversal_reference_rng_statistics():
+    # Reference RNG mode seeds by queue slot (rayqueue.h never copies m_seed), so
+    # one tie resolved differently reorders every later slot: compare statistics.
+    s = gpr.Scene.cornell(128, 128)
+    g, _, _, _ = gpu_render(s, 16, 5, gpr.RNG_REFERENCE, 1, gpr.TRAVERSAL_FAST)
+    o, _ = oracle_render(s, 16, 5, gpr.RNG_REFERENCE, 1)
+    print(f"[render] ref-rng means gpu {g.mean():.6f} oracle {o.mean():.6f}", flush=True)
+    assert abs(g.mean() - o.mean()) <= 0.02 * o.mean()
+
+
+def test_sphere_grid_fast_traversal():
+    s = gpr.Scene.sphere_grid(96, 54, 16)
+    g, _, _, _ = gpu_render(s, 4, 8, gpr.RNG_PCG, 7, gpr.TRAVERSAL_FAST)
+    o, _ = oracle_render(s, 4, 8, gpr.RNG_PCG, 7)
+    frac = rel_close(g, o).mean()
+    print(f"[render] sphere grid fast traversal agreement {frac:.6f}", flush=True)
+    assert frac >= 0.995
+
+
+def test_env_light_and_reset():
+    s = gpr.Scene.cornell(16, 16)
+    r = gpr.Renderer(s, max_depth=3, rng_mode=gpr.RNG_PCG, seed=3)
+    r.render(2)
+    a, c = r.read()
+    assert np.all(c == 2)
+    r.reset()
+    r.render(2)
+    b, c2 = r.read()
+    assert np.all(c2 == 2) and np.array_equal(a, b)  # reset restarts the sample sequence
+    r.free()
