@@ -35,7 +35,20 @@ def parse():
     ap.add_argument("--sort-iters", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays in the CPU baseline sample")
+    ap.add_argument("--only", choices=["all", "trace", "sort", "copy"], default="all",
+                    help="profiling runs: restrict the GPU work to one kernel family")
     return ap.parse_args()
+
+
+def copy_calibration(torch, nbytes=1 << 30, reps=5):
+    """Plain device copy of known size: the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters of a profiling run to bytes."""
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
 
 
 def cpu_baseline(pos, rays_np, sample):
@@ -70,6 +83,9 @@ def main():
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
     gpr.set_stream(stream.cuda_stream)
+    if a.only == "copy":
+        copy_calibration(torch)
+        return
 
     print("[bench] generating inputs", file=sys.stderr, flush=True)
     pos = gpr.gen_soup(2, 0, N_TRI, 0.005)
@@ -90,22 +106,56 @@ def main():
     d_hits = torch.empty(N_RAYS * 16, dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
 
-    # ---- radix sort: mean over back-to-back sorts of the scene's Morton keys
+    if a.only == "trace":
+        for _ in range(a.warmup + a.steps):
+            scene.trace_device(d_rays.data_ptr(), d_hits.data_ptr(), N_RAYS)
+        torch.cuda.synchronize()
+        print(json.dumps({"profiled": "closest-hit", "launches": a.warmup + a.steps}), flush=True)
+        return
+    # ---- radix sort: mean over back-to-back sorts of the scene's Morton keys,
+    # captured as one graph of sort_iters sorts (an even count keeps the
+    # double-buffered digit totals in phase) and replayed
     keys, _, _ = scene.keys()
     d_keys = torch.from_numpy(keys.view(np.int32)).cuda()
     d_sk = torch.empty_like(d_keys)
     d_si = torch.empty_like(d_keys)
-    for _ in range(5):
-        gpr.sort_pairs_device(d_keys.data_ptr(), d_sk.data_ptr(), d_si.data_ptr(), N_TRI)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    gpr.sort_pairs_device(d_keys.data_ptr(), d_sk.data_ptr(), d_si.data_ptr(), N_TRI)
     torch.cuda.synchronize()
+    ref_sk, ref_si = d_sk.clone(), d_si.clone()
+    iters = a.sort_iters + (a.sort_iters & 1)
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    with torch.cuda.stream(cap):
+        gpr.set_stream(cap.cuda_stream)
+        gpr.sort_pairs_device(d_keys.data_ptr(), d_sk.data_ptr(), d_si.data_ptr(), N_TRI)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=cap):
+            for _ in range(iters):
+                gpr.sort_pairs_device(d_keys.data_ptr(), d_sk.data_ptr(), d_si.data_ptr(), N_TRI)
+    gpr.set_stream(stream.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    graph.replay()
+    torch.cuda.synchronize()
+    sort_ok = bool(torch.equal(d_sk, ref_sk) and torch.equal(d_si, ref_si))
+    d_sk.zero_()
     e0.record()
-    for _ in range(a.sort_iters):
+    graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    sort_ms = e0.elapsed_time(e1) / iters
+    sort_ok = sort_ok and bool(torch.equal(d_sk, ref_sk) and torch.equal(d_si, ref_si))
+    # the same sorts launched one by one (no graph), for reference
+    e0.record()
+    for _ in range(iters):
         gpr.sort_pairs_device(d_keys.data_ptr(), d_sk.data_ptr(), d_si.data_ptr(), N_TRI)
     e1.record()
     torch.cuda.synchronize()
-    sort_ms = e0.elapsed_time(e1) / a.sort_iters
+    sort_ms_eager = e0.elapsed_time(e1) / iters
     sort_bytes = 68.0 * N_TRI
+    if a.only == "sort":
+        print(json.dumps({"profiled": "sort", "iters": iters, "ms_per_sort_graph": sort_ms,
+                          "ms_per_sort_eager": sort_ms_eager, "graph_result_ok": sort_ok}), flush=True)
+        return
 
     # ---- traversal statistics of the reference visit order (algorithmic bytes)
     print("[bench] reference-order statistics", file=sys.stderr, flush=True)
@@ -167,8 +217,10 @@ def main():
         "roofline_sort": {"bound": "hbm", "achieved": round(sort_bytes / (sort_ms * 1e-3) / 1e9, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(sort_bytes / (sort_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "ms_per_sort": round(sort_ms, 5), "keys": N_TRI,
-                          "bytes_per_key": 68, "iters": a.sort_iters},
+                          "ms_per_sort": round(sort_ms, 5), "ms_per_sort_eager": round(sort_ms_eager, 5),
+                          "keys": N_TRI, "bytes_per_key": 68, "iters": iters,
+                          "timing": "mean over back-to-back sorts replayed from one hipGraph",
+                          "graph_result_ok": sort_ok},
         "anyhit_mrays_s": round(N_RAYS / (ms_any * 1e-3) / 1e6, 2),
         "anyhit_ms": round(ms_any, 4),
         "closest_hit_fraction": round(hit_frac, 4),

@@ -437,11 +437,17 @@ int gpr_bvh_last_timing(const gpr_scene* s, float ms[5]) {
 
 int gpr_sort_pairs_device(const uint32_t* keys, uint32_t* sk, uint32_t* si, uint32_t n) {
     static DevBuf scratch[16];
+    static uint32_t calls[16];
     if (n == 0) return GPR_OK;
     if (!keys || !sk || !si) return fail(GPR_ERR_INVALID, "gpr_sort_pairs_device: null pointer");
     DevBuf& b = scratch[g_device & 15];
-    GPR_TRY(b.alloc(sort_scratch_bytes(n)));
-    return sort_pairs(keys, sk, si, n, b.p, b.bytes, stream());
+    const size_t need = sort_scratch_bytes(n);
+    if (!b.p || b.bytes < need) {
+        GPR_TRY(b.alloc(need));
+        GPR_TRY(sort_prepare(b.p, b.bytes, stream()));
+        calls[g_device & 15] = 0;
+    }
+    return sort_pairs(keys, sk, si, n, b.p, b.bytes, &calls[g_device & 15], stream());
 }
 
 int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags) {

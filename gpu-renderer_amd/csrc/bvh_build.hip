@@ -23,17 +23,7 @@ namespace gpr {
 namespace {
 
 constexpr int kB = 256;              // threads per block
-constexpr int kTileKeys = 4096;      // keys per sort tile (16 per thread)
-constexpr int kDigits = 1024;        // top 10 of the 30 key bits
-constexpr int kDigitShift = 20;
-constexpr int kLocalCap = 4096;      // bucket size sorted entirely in LDS
 constexpr int kMaxDepth = 256;       // > 30 radix levels + 32 midpoint levels
-
-// Load through the device-coherent path (another XCD's L2 may hold the line).
-template <class T>
-GPR_D T ld_agent(const T* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
@@ -130,30 +120,6 @@ __global__ __launch_bounds__(kB) void k_morton(const float* __restrict__ pos, ui
     keys[i] = (spread3(quant(q.z)) << 2) | (spread3(quant(q.y)) << 1) | spread3(quant(q.x));
 }
 
-// -------------------------------------------------------------- sort ----
-// MSD radix sort specialised for 30-bit keys:
-//   hist    per-tile counts of the top 10 bits           (reads 4 B/key)
-//   scan    per-digit exclusive scan over tiles           (1 MB of counts at 1M keys)
-//   scatter keys into their digit bucket as (key<<32|idx)  (reads 4 B, writes 8 B/key)
-//   local   each bucket sorted as 64-bit composites in LDS (reads 8 B, writes 8 B/key)
-// Sorting the composite makes the result independent of the order inside a
-// bucket, so the scatter may use LDS atomics, and ties come out in index order.
-__global__ __launch_bounds__(kB) void k_sort_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                  uint32_t* __restrict__ counts) {
-    __shared__ uint32_t h[kDigits];
-    for (int d = threadIdx.x; d < kDigits; d += kB) h[d] = 0;
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kTileKeys;
-#pragma unroll 4
-    for (int j = 0; j < kTileKeys / kB; ++j) {
-        const uint64_t i = base + j * kB + threadIdx.x;
-        if (i < n) atomicAdd(&h[keys[i] >> kDigitShift], 1u);
-    }
-    __syncthreads();
-    uint32_t* row = counts + (uint64_t)blockIdx.x * kDigits;
-    for (int d = threadIdx.x; d < kDigits; d += kB) row[d] = h[d];
-}
-
 // Inclusive block scan (256 threads, wave64) of one value per thread.
 GPR_D uint32_t block_incl_scan(uint32_t v, uint32_t* sh /* >= 4 */, uint32_t* total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -174,162 +140,6 @@ GPR_D uint32_t block_incl_scan(uint32_t v, uint32_t* sh /* >= 4 */, uint32_t* to
     __syncthreads();
     *total = tot;
     return v + add;
-}
-
-__global__ __launch_bounds__(kB) void k_sort_scan(const uint32_t* __restrict__ counts, uint32_t ntiles,
-                                                  uint32_t* __restrict__ offs,
-                                                  uint32_t* __restrict__ totals) {
-    __shared__ uint32_t sh[4];
-    const uint32_t d = blockIdx.x;
-    uint32_t carry = 0;
-    for (uint32_t t0 = 0; t0 < ntiles; t0 += kB) {
-        const uint32_t t = t0 + threadIdx.x;
-        const uint32_t c = t < ntiles ? counts[(uint64_t)t * kDigits + d] : 0u;
-        uint32_t tot;
-        const uint32_t inc = block_incl_scan(c, sh, &tot);
-        if (t < ntiles) offs[(uint64_t)t * kDigits + d] = carry + inc - c;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) totals[d] = carry;
-}
-
-// Exclusive scan of the 1024 digit totals into LDS (4 per thread).
-GPR_D void digit_bases(const uint32_t* __restrict__ totals, uint32_t* base, uint32_t* sh) {
-    const int t = threadIdx.x;
-    uint32_t v[4], s = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        v[k] = totals[4 * t + k];
-        s += v[k];
-    }
-    uint32_t tot;
-    uint32_t run = block_incl_scan(s, sh, &tot) - s;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        base[4 * t + k] = run;
-        run += v[k];
-    }
-    __syncthreads();
-}
-
-__global__ __launch_bounds__(kB) void k_sort_scatter(const uint32_t* __restrict__ keys, uint32_t n,
-                                                     const uint32_t* __restrict__ offs,
-                                                     const uint32_t* __restrict__ totals,
-                                                     unsigned long long* __restrict__ tmp) {
-    __shared__ uint32_t pos[kDigits];
-    __shared__ uint32_t sh[4];
-    digit_bases(totals, pos, sh);
-    const uint32_t* row = offs + (uint64_t)blockIdx.x * kDigits;
-    for (int d = threadIdx.x; d < kDigits; d += kB) pos[d] += row[d];
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kTileKeys;
-#pragma unroll 4
-    for (int j = 0; j < kTileKeys / kB; ++j) {
-        const uint64_t i = base + j * kB + threadIdx.x;
-        if (i < n) {
-            const uint32_t k = keys[i];
-            const uint32_t slot = atomicAdd(&pos[k >> kDigitShift], 1u);
-            tmp[slot] = ((unsigned long long)k << 32) | (unsigned long long)i;
-        }
-    }
-}
-
-GPR_D uint32_t lower_bound_u64(const unsigned long long* a, uint32_t n, unsigned long long v) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Bitonic sort of P (power of two) 64-bit values in LDS.
-GPR_D void lds_bitonic(unsigned long long* s, uint32_t P) {
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < (P >> 1); i += kB) {
-                const uint32_t a = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-                const uint32_t b = a + j;
-                const unsigned long long x = s[a], y = s[b];
-                const bool up = (a & k) == 0;
-                if ((x > y) == up) {
-                    s[a] = y;
-                    s[b] = x;
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-__global__ __launch_bounds__(kB) void k_sort_local(const uint32_t* __restrict__ totals,
-                                                   unsigned long long* __restrict__ tmp,
-                                                   unsigned long long* __restrict__ alt,
-                                                   uint32_t* __restrict__ skeys,
-                                                   uint32_t* __restrict__ sidx) {
-    __shared__ unsigned long long s[kLocalCap];
-    __shared__ uint32_t red[kB];
-    const uint32_t d = blockIdx.x;
-    // bucket start = sum of totals[0..d)
-    uint32_t part = 0;
-    for (uint32_t i = threadIdx.x; i < d; i += kB) part += totals[i];
-    red[threadIdx.x] = part;
-    __syncthreads();
-    for (int st = kB / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
-    }
-    const uint32_t b0 = red[0];
-    const uint32_t n = totals[d];
-    if (n == 0) return;
-    unsigned long long* src = tmp + b0;
-    if (n <= kLocalCap) {
-        uint32_t P = 2;
-        while (P < n) P <<= 1;
-        for (uint32_t i = threadIdx.x; i < P; i += kB) s[i] = i < n ? src[i] : ~0ull;
-        __syncthreads();
-        lds_bitonic(s, P);
-        for (uint32_t i = threadIdx.x; i < n; i += kB) {
-            const unsigned long long v = s[i];
-            skeys[b0 + i] = (uint32_t)(v >> 32);
-            sidx[b0 + i] = (uint32_t)v;
-        }
-        return;
-    }
-    // Oversized bucket (skewed key distributions): sort LDS-sized chunks, then
-    // merge runs pairwise through `alt`; each element finds its output rank by
-    // a binary search in the sibling run (composites are unique).
-    for (uint32_t c0 = 0; c0 < n; c0 += kLocalCap) {
-        const uint32_t m = min((uint32_t)kLocalCap, n - c0);
-        for (uint32_t i = threadIdx.x; i < kLocalCap; i += kB) s[i] = i < m ? src[c0 + i] : ~0ull;
-        __syncthreads();
-        lds_bitonic(s, kLocalCap);
-        for (uint32_t i = threadIdx.x; i < m; i += kB) src[c0 + i] = s[i];
-        __syncthreads();
-    }
-    unsigned long long* a = src;
-    unsigned long long* bb = alt + b0;
-    for (uint32_t w = kLocalCap; w < n; w <<= 1) {
-        for (uint32_t e = threadIdx.x; e < n; e += kB) {
-            const uint32_t r0 = (e / (2 * w)) * (2 * w);
-            const uint32_t mid = min(r0 + w, n), end = min(r0 + 2 * w, n);
-            const unsigned long long v = a[e];
-            uint32_t out;
-            if (e < mid) out = (e - r0) + lower_bound_u64(a + mid, end - mid, v);
-            else out = (e - mid) + lower_bound_u64(a + r0, mid - r0, v);
-            bb[r0 + out] = v;
-        }
-        __syncthreads();
-        unsigned long long* t = a;
-        a = bb;
-        bb = t;
-    }
-    for (uint32_t i = threadIdx.x; i < n; i += kB) {
-        const unsigned long long v = a[i];
-        skeys[b0 + i] = (uint32_t)(v >> 32);
-        sidx[b0 + i] = (uint32_t)v;
-    }
 }
 
 // ---------------------------------------------------------- weights -----
@@ -383,6 +193,20 @@ __global__ __launch_bounds__(kB) void k_weights(const uint32_t* __restrict__ sk,
 // leaf i = (n-1)+i.  A node covering [l, r] has parent r (as left child) if
 // the split after r is deeper than the split before l, else parent l-1 (as
 // right child).  The second child to arrive at a parent forms its box.
+//
+// Everything one thread publishes for another (boxes, child ids, range
+// bounds) goes through device-scope atomic read-modify-writes, which are
+// performed at the device-coherent level; the writer waits for their
+// completion before it bumps the arrival counter and the reader only loads
+// (again by RMW) after seeing that counter.  No L2 write-back fence is needed
+// although the eight XCD L2 caches are not coherent with each other.
+GPR_D void pub(uint32_t* p, uint32_t v) {
+    (void)__hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+GPR_D uint32_t get(uint32_t* p) {
+    return __hip_atomic_fetch_or(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __global__ __launch_bounds__(kB) void k_tree(const float* __restrict__ pos, uint32_t n,
                                              const uint32_t* __restrict__ sidx,
                                              const int32_t* __restrict__ w,
@@ -397,12 +221,16 @@ __global__ __launch_bounds__(kB) void k_tree(const float* __restrict__ pos, uint
     v3 lo, hi;
     tri_bounds(pos + 9ull * sidx[i], lo, hi);
     uint32_t node = (n - 1) + i;
-    {
-        float* b = box + 6ull * node;
-        b[0] = lo.x; b[1] = lo.y; b[2] = lo.z; b[3] = hi.x; b[4] = hi.y; b[5] = hi.z;
-    }
+    uint32_t* bx = reinterpret_cast<uint32_t*>(box);
     uint32_t l = i, r = i;
     for (int guard = 0; guard < kMaxDepth; ++guard) {
+        uint32_t* b = bx + 6ull * node;
+        pub(b + 0, __float_as_uint(lo.x));
+        pub(b + 1, __float_as_uint(lo.y));
+        pub(b + 2, __float_as_uint(lo.z));
+        pub(b + 3, __float_as_uint(hi.x));
+        pub(b + 4, __float_as_uint(hi.y));
+        pub(b + 5, __float_as_uint(hi.z));
         if (l == 0 && r == n - 1) {
             *root = node;
             break;
@@ -414,31 +242,29 @@ __global__ __launch_bounds__(kB) void k_tree(const float* __restrict__ pos, uint
         const uint32_t p = as_left ? r : l - 1;
         parent[node] = p;
         if (as_left) {
-            child[2 * p] = node;
-            atomicExch(&rlo[p], l);
+            pub(&child[2 * p], node);
+            pub(&rlo[p], l);
         } else {
-            child[2 * p + 1] = node;
-            atomicExch(&rhi[p], r);
+            pub(&child[2 * p + 1], node);
+            pub(&rhi[p], r);
         }
-        __threadfence();
-        if (atomicAdd(&arrive[p], 1u) == 0) break;
-        __threadfence();
-        // second arrival: both children published
-        if (as_left) r = ld_agent(&rhi[p]);
-        else l = ld_agent(&rlo[p]);
-        const uint32_t lc = as_left ? node : ld_agent(&child[2 * p]);
-        const uint32_t rc = as_left ? ld_agent(&child[2 * p + 1]) : node;
-        const float* bl = box + 6ull * lc;
-        const float* br = box + 6ull * rc;
+        __builtin_amdgcn_s_waitcnt(0);  // every publish above has completed
+        if (__hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+        // second arrival: the sibling's publishes completed before its increment
+        if (as_left) r = get(&rhi[p]);
+        else l = get(&rlo[p]);
+        const uint32_t sib = get(&child[2 * p + (as_left ? 1 : 0)]);
+        uint32_t* sb = bx + 6ull * sib;
+        const v3 slo = V(__uint_as_float(get(sb + 0)), __uint_as_float(get(sb + 1)), __uint_as_float(get(sb + 2)));
+        const v3 shi = V(__uint_as_float(get(sb + 3)), __uint_as_float(get(sb + 4)), __uint_as_float(get(sb + 5)));
         // Union(l_child, r_child) (aabb.h:37-46, bvh.h:182)
-        const v3 llo = V(ld_agent(bl + 0), ld_agent(bl + 1), ld_agent(bl + 2));
-        const v3 lhi = V(ld_agent(bl + 3), ld_agent(bl + 4), ld_agent(bl + 5));
-        const v3 rlo_ = V(ld_agent(br + 0), ld_agent(br + 1), ld_agent(br + 2));
-        const v3 rhi_ = V(ld_agent(br + 3), ld_agent(br + 4), ld_agent(br + 5));
-        lo = vmin3(llo, rlo_);
-        hi = vmax3(lhi, rhi_);
-        float* b = box + 6ull * p;
-        b[0] = lo.x; b[1] = lo.y; b[2] = lo.z; b[3] = hi.x; b[4] = hi.y; b[5] = hi.z;
+        if (as_left) {
+            lo = vmin3(lo, slo);
+            hi = vmax3(hi, shi);
+        } else {
+            lo = vmin3(slo, lo);
+            hi = vmax3(shi, hi);
+        }
         node = p;
     }
 }
@@ -608,44 +434,7 @@ struct Carve {
     }
 };
 
-struct SortLayout {
-    uint32_t* counts;
-    uint32_t* offs;
-    uint32_t* totals;
-    unsigned long long* tmp;
-    unsigned long long* alt;
-};
-size_t sort_layout(uint32_t n, char* base, SortLayout* L) {
-    const uint32_t ntiles = std::max(1u, div_up(n, kTileKeys));
-    Carve c{base, 0};
-    SortLayout l;
-    l.counts = c.take<uint32_t>((size_t)ntiles * kDigits);
-    l.offs = c.take<uint32_t>((size_t)ntiles * kDigits);
-    l.totals = c.take<uint32_t>(kDigits);
-    l.tmp = c.take<unsigned long long>(std::max(1u, n));
-    l.alt = c.take<unsigned long long>(std::max(1u, n));
-    if (L) *L = l;
-    return c.off + 256;
-}
-
 }  // namespace
-
-size_t sort_scratch_bytes(uint32_t n) { return sort_layout(n, nullptr, nullptr); }
-
-int sort_pairs(const uint32_t* keys, uint32_t* skeys, uint32_t* sidx, uint32_t n, void* scratch,
-               size_t scratch_bytes, hipStream_t st) {
-    if (n == 0) return GPR_OK;
-    if (scratch_bytes < sort_scratch_bytes(n)) return fail(GPR_ERR_INVALID, "sort scratch too small");
-    SortLayout L;
-    sort_layout(n, reinterpret_cast<char*>(scratch), &L);
-    const uint32_t ntiles = div_up(n, kTileKeys);
-    hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kB), 0, st, keys, n, L.counts);
-    hipLaunchKernelGGL(k_sort_scan, dim3(kDigits), dim3(kB), 0, st, L.counts, ntiles, L.offs, L.totals);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kB), 0, st, keys, n, L.offs, L.totals, L.tmp);
-    hipLaunchKernelGGL(k_sort_local, dim3(kDigits), dim3(kB), 0, st, L.totals, L.tmp, L.alt, skeys, sidx);
-    GPR_HIP(hipGetLastError());
-    return GPR_OK;
-}
 
 int build_bvh(gpr_scene* s) {
     BvhDev& B = s->bvh;
@@ -668,7 +457,6 @@ int build_bvh(gpr_scene* s) {
         uint32_t* keys = c.take<uint32_t>(n);
         uint32_t* skeys = c.take<uint32_t>(n);
         uint32_t* sidx = c.take<uint32_t>(n);
-        char* sortmem = c.take<char>(sort_scratch_bytes(n));
         int32_t* w = c.take<int32_t>(n);
         uint32_t* parent = c.take<uint32_t>(nn);
         uint32_t* child = c.take<uint32_t>(2ull * n);
@@ -689,6 +477,12 @@ int build_bvh(gpr_scene* s) {
         if (pass == 0) {
             need = c.off + 256;
             GPR_TRY(B.arena.alloc(need));
+            const size_t sb = sort_scratch_bytes(n);
+            if (B.sort_scratch.bytes < sb || !B.sort_scratch.p) {
+                GPR_TRY(B.sort_scratch.alloc(sb));
+                GPR_TRY(sort_prepare(B.sort_scratch.p, B.sort_scratch.bytes, st));
+                B.sort_calls = 0;
+            }
             continue;
         }
         B.keys = keys;
@@ -705,7 +499,7 @@ int build_bvh(gpr_scene* s) {
         hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(kB), 0, st, part, npart, bounds);
         hipLaunchKernelGGL(k_morton, dim3(div_up(n, kB)), dim3(kB), 0, st, pos, n, bounds, keys);
         GPR_HIP(hipEventRecord(ev[1], st));
-        GPR_TRY(sort_pairs(keys, skeys, sidx, n, sortmem, sort_scratch_bytes(n), st));
+        GPR_TRY(sort_pairs(keys, skeys, sidx, n, B.sort_scratch.p, B.sort_scratch.bytes, &B.sort_calls, st));
         GPR_HIP(hipEventRecord(ev[2], st));
         hipLaunchKernelGGL(k_fill_u32, dim3(std::min(1024u, div_up(n, kB))), dim3(kB), 0, st, arrive, 0u, (uint64_t)n);
         hipLaunchKernelGGL(k_weights, dim3(div_up(n, kB)), dim3(kB), 0, st, skeys, n, w);

@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
     uint32_t pix = 0;
     unsigned long long rng_out = 0;
     if (valid) {
-        const float4 ot = qi.o_tmin[i], dt = qi.d_tmax[i], hh = qi.hit[i], wp = qi.w_pdf[i];
+        const float4 dt = qi.d_tmax[i], hh = qi.hit[i], wp = qi.w_pdf[i];
         const uint32_t pw = qi.pix[i];
         pix = pw & 0x7fffffffu;
         const bool spec = (pw >> 31) != 0;

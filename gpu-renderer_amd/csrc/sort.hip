@@ -1,0 +1,355 @@
+// sort.hip -- LDS-staged radix sort of 30-bit Morton keys into (key, index)
+// pairs for gfx950 (the std::sort of LBVH_build, src/renderer/bvh.h:207-208,
+// defined stable: equal keys keep index order).
+//
+// The top digit has between 1 and 11 bits so that a bucket holds ~1K keys
+// (10 bits = 1024 buckets at 1M keys).  Three launches, each reading its
+// input once:
+//   hist     16384-key tiles (1024 threads): LDS histograms of the top digit
+//            per 4096-key quarter; every tile reserves its range inside each
+//            bucket with one returning atomic per non-empty digit and hands
+//            the quarters consecutive sub-ranges.  The atomics make the tile
+//            order inside a bucket arbitrary, which is harmless because the
+//            last pass orders the full (key, index).
+//   scatter  4096-key quarters: bucket bases from an LDS scan of the digit
+//            totals, LDS-atomic ranks inside the quarter, 8-byte
+//            (key<<32 | index) composites written into their bucket.
+//   local    one block per bucket staged in LDS: MSD split on the next 10 key
+//            bits into 1024 sub-buckets (~1 element each), then an insertion
+//            sort of every sub-bucket on the full composite.  Buckets with a
+//            crowded sub-bucket, or too large for LDS, use LDS bitonic chunk
+//            sorts and pairwise merges.
+// The digit-total buffer is double-buffered across calls: each call clears
+// the one the next call uses.
+#include "dev_math.h"
+#include "gpr_internal.h"
+
+#include <algorithm>
+
+namespace gpr {
+namespace {
+
+constexpr int kMaxDigits = 2048;    // top-digit buckets (11 bits at most)
+constexpr int kQuarters = 4;
+constexpr int kTile = 16384;        // keys per histogram tile
+constexpr int kSubTile = kTile / kQuarters;  // keys per scatter tile
+constexpr int kTileThreads = 1024;
+constexpr int kLocalThreads = 256;
+constexpr int kFastCap = 1536;      // bucket size of the LDS fast path (A and B)
+constexpr int kSub = 1024;          // sub-buckets of the LDS MSD split
+constexpr int kSubMax = 48;         // largest sub-bucket sorted by one thread
+
+// Top-digit width: ~1K keys per bucket, between 1 and 11 bits.
+uint32_t digit_bits(uint32_t n) {
+    uint32_t b = 1;
+    while (b < 11u && ((uint64_t)n >> b) > 1024u) ++b;
+    return b;
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// inclusive scan of one value per thread over a block of NT threads (wave64)
+template <int NT>
+GPR_D uint32_t scan_incl(uint32_t v, uint32_t* sh, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    if (lane == 63) sh[wave] = v;
+    __syncthreads();
+    uint32_t add = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+        const uint32_t s = sh[w];
+        add += w < wave ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return v + add;
+}
+
+// ---- hist: 16384-key tiles, per-quarter histograms, one reservation per digit
+__global__ __launch_bounds__(kTileThreads) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                       uint32_t shift, uint32_t nd,
+                                                       uint32_t* __restrict__ acc,
+                                                       uint32_t* __restrict__ acc_next,
+                                                       uint32_t* __restrict__ off) {
+    __shared__ uint32_t h[kQuarters * kMaxDigits];
+    for (uint32_t i = threadIdx.x; i < kQuarters * nd; i += kTileThreads) h[i] = 0;
+    if (blockIdx.x == 0)
+        for (uint32_t d = threadIdx.x; d < nd; d += kTileThreads) acc_next[d] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    uint32_t k[kTile / kTileThreads];
+#pragma unroll
+    for (int j = 0; j < kTile / kTileThreads; ++j) {
+        const uint64_t i = base + j * kTileThreads + threadIdx.x;
+        k[j] = i < n ? keys[i] : 0xffffffffu;
+    }
+#pragma unroll
+    for (int j = 0; j < kTile / kTileThreads; ++j) {
+        // element j*1024+t lies in quarter (j*1024+t)/4096 = j/4
+        if (k[j] != 0xffffffffu) atomicAdd(&h[(j / 4) * nd + ((k[j] >> shift) & (nd - 1))], 1u);
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < nd; d += kTileThreads) {
+        uint32_t q[kQuarters], tot = 0;
+#pragma unroll
+        for (int x = 0; x < kQuarters; ++x) {
+            q[x] = h[x * nd + d];
+            tot += q[x];
+        }
+        uint32_t r = tot ? atomicAdd(&acc[d], tot) : 0u;
+#pragma unroll
+        for (int x = 0; x < kQuarters; ++x) {
+            off[((uint64_t)blockIdx.x * kQuarters + x) * nd + d] = r;
+            r += q[x];
+        }
+    }
+}
+
+// ---- scatter: 4096-key sub-tiles, composites into their bucket
+__global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __restrict__ keys, uint32_t n,
+                                                          uint32_t shift, uint32_t nd,
+                                                          const uint32_t* __restrict__ acc,
+                                                          const uint32_t* __restrict__ off,
+                                                          uint32_t* __restrict__ bases,
+                                                          unsigned long long* __restrict__ tmp) {
+    __shared__ uint32_t pos[kMaxDigits];
+    __shared__ uint32_t sh[kTileThreads / 64];
+    const uint64_t b0 = (uint64_t)blockIdx.x * kSubTile;
+    uint32_t k[kSubTile / kTileThreads];
+#pragma unroll
+    for (int j = 0; j < kSubTile / kTileThreads; ++j) {
+        const uint64_t i = b0 + j * kTileThreads + threadIdx.x;
+        k[j] = i < n ? keys[i] : 0u;
+    }
+    // bucket bases: exclusive scan of the digit totals (nd/1024 per thread)
+    const uint32_t per = nd / kTileThreads;  // nd is a multiple of 1024 or below it
+    if (per <= 1) {
+        const uint32_t v = threadIdx.x < nd ? acc[threadIdx.x] : 0u;
+        uint32_t tot;
+        const uint32_t ex = scan_incl<kTileThreads>(v, sh, &tot) - v;
+        if (threadIdx.x < nd) pos[threadIdx.x] = ex;
+    } else {
+        uint32_t v[2] = {acc[2 * threadIdx.x], acc[2 * threadIdx.x + 1]};
+        uint32_t tot;
+        const uint32_t ex = scan_incl<kTileThreads>(v[0] + v[1], sh, &tot) - (v[0] + v[1]);
+        pos[2 * threadIdx.x] = ex;
+        pos[2 * threadIdx.x + 1] = ex + v[0];
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < nd; d += kTileThreads) {
+        const uint32_t bd = pos[d];
+        if (blockIdx.x == 0) bases[d] = bd;
+        pos[d] = bd + off[(uint64_t)blockIdx.x * nd + d];
+    }
+    __syncthreads();
+    uint32_t slot[kSubTile / kTileThreads];
+#pragma unroll
+    for (int j = 0; j < kSubTile / kTileThreads; ++j) {
+        const uint64_t i = b0 + j * kTileThreads + threadIdx.x;
+        slot[j] = i < n ? atomicAdd(&pos[(k[j] >> shift) & (nd - 1)], 1u) : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kSubTile / kTileThreads; ++j) {
+        const uint64_t i = b0 + j * kTileThreads + threadIdx.x;
+        if (i < n) tmp[slot[j]] = ((unsigned long long)k[j] << 32) | (unsigned long long)i;
+    }
+}
+
+GPR_D void bitonic_lds(unsigned long long* s, uint32_t P) {
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < (P >> 1); i += kLocalThreads) {
+                const uint32_t a = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                const uint32_t b = a + j;
+                const unsigned long long x = s[a], y = s[b];
+                if ((x > y) == ((a & k) == 0)) {
+                    s[a] = y;
+                    s[b] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+GPR_D uint32_t lower_bound_u64(const unsigned long long* a, uint32_t n, unsigned long long v) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---- local: one bucket per block, LDS MSD split + insertion sort
+__global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const uint32_t* __restrict__ acc,
+                                                         const uint32_t* __restrict__ bases,
+                                                         unsigned long long* __restrict__ tmp,
+                                                         unsigned long long* __restrict__ alt,
+                                                         uint32_t* __restrict__ skeys,
+                                                         uint32_t* __restrict__ sidx) {
+    __shared__ unsigned long long buf[2 * kFastCap];  // A | B
+    __shared__ uint32_t cnt[kSub];
+    __shared__ uint32_t sh[kLocalThreads / 64];
+    __shared__ uint32_t crowded;
+    const uint32_t d = blockIdx.x;
+    const uint32_t n = acc[d];
+    if (n == 0) return;
+    const uint32_t b0 = bases[d];
+    unsigned long long* src = tmp + b0;
+    const uint32_t sub_bits = min(10u, shift);
+    const uint32_t nsub = 1u << sub_bits;
+    const uint32_t sub_shift = 32 + shift - sub_bits;
+    if (n <= (uint32_t)kFastCap) {
+        unsigned long long* A = buf;
+        unsigned long long* B = buf + kFastCap;
+        for (uint32_t s = threadIdx.x; s < kSub; s += kLocalThreads) cnt[s] = 0;
+        if (threadIdx.x == 0) crowded = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += kLocalThreads) {
+            const unsigned long long v = src[i];
+            A[i] = v;
+            atomicAdd(&cnt[(uint32_t)(v >> sub_shift) & (nsub - 1)], 1u);
+        }
+        __syncthreads();
+        uint32_t c4[4], s4 = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            c4[k] = cnt[4 * threadIdx.x + k];
+            s4 += c4[k];
+            if (c4[k] > kSubMax) crowded = 1;
+        }
+        uint32_t tot;
+        uint32_t run = scan_incl<kLocalThreads>(s4, sh, &tot) - s4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            cnt[4 * threadIdx.x + k] = run;
+            run += c4[k];
+        }
+        __syncthreads();
+        if (!crowded) {
+            for (uint32_t i = threadIdx.x; i < n; i += kLocalThreads) {
+                const unsigned long long v = A[i];
+                B[atomicAdd(&cnt[(uint32_t)(v >> sub_shift) & (nsub - 1)], 1u)] = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t e = cnt[4 * threadIdx.x + k], b = e - c4[k];
+                for (uint32_t x = b + 1; x < e; ++x) {
+                    const unsigned long long v = B[x];
+                    uint32_t y = x;
+                    while (y > b && B[y - 1] > v) {
+                        B[y] = B[y - 1];
+                        --y;
+                    }
+                    B[y] = v;
+                }
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += kLocalThreads) {
+                const unsigned long long v = B[i];
+                skeys[b0 + i] = (uint32_t)(v >> 32);
+                sidx[b0 + i] = (uint32_t)v;
+            }
+            return;
+        }
+    }
+    // crowded or oversized bucket: LDS chunk sorts, then pairwise merges through `alt`
+    constexpr uint32_t kChunk = 2 * kFastCap >= 2048 ? 2048 : 1024;
+    for (uint32_t c0 = 0; c0 < n; c0 += kChunk) {
+        const uint32_t m = min(kChunk, n - c0);
+        for (uint32_t i = threadIdx.x; i < kChunk; i += kLocalThreads) buf[i] = i < m ? src[c0 + i] : ~0ull;
+        __syncthreads();
+        bitonic_lds(buf, kChunk);
+        for (uint32_t i = threadIdx.x; i < m; i += kLocalThreads) src[c0 + i] = buf[i];
+        __syncthreads();
+    }
+    unsigned long long* x = src;
+    unsigned long long* y = alt + b0;
+    for (uint32_t w = kChunk; w < n; w <<= 1) {
+        for (uint32_t e = threadIdx.x; e < n; e += kLocalThreads) {
+            const uint32_t r0 = (e / (2 * w)) * (2 * w);
+            const uint32_t mid = min(r0 + w, n), end = min(r0 + 2 * w, n);
+            const unsigned long long v = x[e];
+            const uint32_t o = e < mid ? (e - r0) + lower_bound_u64(x + mid, end - mid, v)
+                                       : (e - mid) + lower_bound_u64(x + r0, mid - r0, v);
+            y[r0 + o] = v;
+        }
+        __syncthreads();
+        unsigned long long* t = x;
+        x = y;
+        y = t;
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += kLocalThreads) {
+        const unsigned long long v = x[i];
+        skeys[b0 + i] = (uint32_t)(v >> 32);
+        sidx[b0 + i] = (uint32_t)v;
+    }
+}
+
+struct Layout {
+    uint32_t* acc[2];
+    uint32_t* bases;
+    uint32_t* off;
+    unsigned long long* tmp;
+    unsigned long long* alt;
+};
+size_t layout(uint32_t n, char* base, Layout* L) {
+    const uint32_t ntiles = std::max(1u, div_up(n, kTile));
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        char* p = base ? base + off : nullptr;
+        off += bytes;
+        return p;
+    };
+    Layout l;
+    l.acc[0] = reinterpret_cast<uint32_t*>(take(4 * kMaxDigits));
+    l.acc[1] = reinterpret_cast<uint32_t*>(take(4 * kMaxDigits));
+    l.bases = reinterpret_cast<uint32_t*>(take(4 * kMaxDigits));
+    l.off = reinterpret_cast<uint32_t*>(take(4ull * kMaxDigits * kQuarters * ntiles));
+    l.tmp = reinterpret_cast<unsigned long long*>(take(8ull * std::max(1u, n)));
+    l.alt = reinterpret_cast<unsigned long long*>(take(8ull * std::max(1u, n)));
+    if (L) *L = l;
+    return off + 256;
+}
+
+}  // namespace
+
+size_t sort_scratch_bytes(uint32_t n) { return layout(n, nullptr, nullptr); }
+
+int sort_prepare(void* scratch, size_t bytes, hipStream_t st) {
+    // both digit-total buffers start cleared; afterwards every call clears the next one
+    GPR_HIP(hipMemsetAsync(scratch, 0, std::min<size_t>(bytes, 2 * 4 * kMaxDigits + 512), st));
+    return GPR_OK;
+}
+
+int sort_pairs(const uint32_t* keys, uint32_t* skeys, uint32_t* sidx, uint32_t n, void* scratch,
+               size_t scratch_bytes, uint32_t* calls, hipStream_t st) {
+    if (n == 0) return GPR_OK;
+    if (scratch_bytes < sort_scratch_bytes(n)) return fail(GPR_ERR_INVALID, "sort scratch too small");
+    Layout L;
+    layout(n, reinterpret_cast<char*>(scratch), &L);
+    const uint32_t par = (*calls)++ & 1u;
+    const uint32_t dbits = digit_bits(n);
+    const uint32_t nd = 1u << dbits, shift = 30 - dbits;
+    const uint32_t ntiles = div_up(n, kTile);
+    hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kTileThreads), 0, st, keys, n, shift, nd, L.acc[par],
+                       L.acc[par ^ 1], L.off);
+    hipLaunchKernelGGL(k_scatter, dim3(div_up(n, kSubTile)), dim3(kTileThreads), 0, st, keys, n, shift, nd,
+                       L.acc[par], L.off, L.bases, L.tmp);
+    hipLaunchKernelGGL(k_local, dim3(nd), dim3(kLocalThreads), 0, st, shift, L.acc[par], L.bases, L.tmp,
+                       L.alt, skeys, sidx);
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+
+}  // namespace gpr

@@ -152,8 +152,9 @@ int gpr_bvh_export_ref_layout(const gpr_scene* s, gpr_ref_node* out, size_t cap)
  * [0] bounds+morton, [1] sort, [2] tree+refit, [3] layouts, [4] total. */
 int gpr_bvh_last_timing(const gpr_scene* s, float ms[5]);
 
-/* Standalone LDS-staged radix sort of n 30-bit keys into (key, index) pairs,
- * stable (ties in index order).  Device pointers. */
+/* Standalone LDS-staged radix sort of n Morton keys into (key, index) pairs,
+ * stable (ties in index order), the sort of LBVH_build (bvh.h:207-208).
+ * Device pointers; keys must be below 2^30 (30-bit Morton codes). */
 int gpr_sort_pairs_device(const uint32_t* keys, uint32_t* sorted_keys, uint32_t* sorted_idx,
                           uint32_t n);
 
