@@ -285,6 +285,7 @@ int gpr_scene_add_mesh(gpr_scene* s, const float* p, uint32_t np, const float* n
     m.ntri = ntri;
     m.mat = *mat;
     s->meshes.push_back(std::move(m));
+    s->flattened = false;
     return GPR_OK;
 }
 
@@ -315,9 +316,12 @@ int gpr_scene_set_env(gpr_scene* s, const float rgb[3]) {
     return GPR_OK;
 }
 
-int gpr_scene_commit(gpr_scene* s) {
-    if (!s) return fail(GPR_ERR_INVALID, "gpr_scene_commit: null scene");
-    if (s->committed) return GPR_OK;
+// Host half of Scene::Scene (src/renderer/scene.cpp:7-20): triangles in mesh
+// order then face order, per-triangle corner data gathered through the
+// (v, vt, vn) corner table, emissive triangles collected as the light list.
+int gpr_scene_flatten(gpr_scene* s) {
+    if (!s) return fail(GPR_ERR_INVALID, "gpr_scene_flatten: null scene");
+    if (s->flattened) return GPR_OK;
     uint64_t total = 0;
     for (const Mesh& m : s->meshes) total += m.ntri;
     if (total >= (1ull << 31)) return fail(GPR_ERR_UNSUPPORTED, "more than 2^31-1 triangles");
@@ -350,6 +354,14 @@ int gpr_scene_commit(gpr_scene* s) {
             if (!(e[0] == 0 && e[1] == 0 && e[2] == 0)) s->lights.push_back((uint32_t)t);
         }
     }
+    s->flattened = true;
+    return GPR_OK;
+}
+
+int gpr_scene_commit(gpr_scene* s) {
+    if (!s) return fail(GPR_ERR_INVALID, "gpr_scene_commit: null scene");
+    if (s->committed) return GPR_OK;
+    GPR_TRY(gpr_scene_flatten(s));
     GPR_HIP(hipSetDevice(s->device));
     GPR_TRY(upload(s->d_pos, s->pos));
     GPR_TRY(upload(s->d_nrm, s->nrm));
@@ -393,7 +405,7 @@ int gpr_scene_get_info(const gpr_scene* s, gpr_scene_info* o) {
 
 int gpr_scene_export_flat(const gpr_scene* s, float* pos, float* nrm, float* uv, uint8_t* flags,
                           uint32_t* mat, gpr_material* mats) {
-    if (!s || !s->committed) return fail(GPR_ERR_INVALID, "gpr_scene_export_flat: scene not committed");
+    if (!s || !s->flattened) return fail(GPR_ERR_INVALID, "gpr_scene_export_flat: scene not flattened or committed");
     if (pos) std::memcpy(pos, s->pos.data(), s->pos.size() * 4);
     if (nrm) std::memcpy(nrm, s->nrm.data(), s->nrm.size() * 4);
     if (uv) std::memcpy(uv, s->uv.data(), s->uv.size() * 4);

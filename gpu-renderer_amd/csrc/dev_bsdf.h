@@ -337,15 +337,18 @@ GPR_D BSample bsdf_sample(const gpr_material& m, const Frame& fr, v3 wo, float u
         const float phi = kPi * 2 * u1;
         const v3 wh = V(st * cos_f(phi), st * sin_f(phi), ct);
         const float dwo = dot3(lo, wh);
-        if (dwo != 0) {
+        // only microfacets facing wo scatter it; a direction on the wrong side
+        // for its lobe has no counterpart in eval/pdf and is dropped
+        if (dwo * lo.z > 0) {
             const float F = fresnel_diel(dwo, 1.f, m.ior);
             bool ok = true;
             if (u2 < F) {
                 li = reflect_v(lo, wh);
+                ok = li.z * lo.z > 0;
             } else {
                 const float eta = lo.z > 0 ? m.ior : 1.f / m.ior;
                 const v3 n = dwo < 0 ? -wh : wh;
-                ok = refract_v(lo, n, 1.f / eta, li);
+                ok = refract_v(lo, n, 1.f / eta, li) && li.z * lo.z < 0;
             }
             if (ok) rdiel_eval(m, lo, li, s.f, s.pdf);
         }

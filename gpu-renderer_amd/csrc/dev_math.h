@@ -70,22 +70,23 @@ GPR_HD void sincos_d(float x, double* s, double* c) {
     double r = xd - q * 1.5707963267948966;
     r = r - q * 6.123233995736766e-17;
     const double r2 = r * r;
-    double ps = 1.0 / 1307674368000.0;
-    ps = -1.0 / 6227020800.0 + r2 * ps;
-    ps = 1.0 / 39916800.0 + r2 * ps;
-    ps = -1.0 / 362880.0 + r2 * ps;
-    ps = 1.0 / 5040.0 + r2 * ps;
-    ps = -1.0 / 120.0 + r2 * ps;
-    ps = 1.0 / 6.0 - r2 * ps;
-    const double sr = r - r * r2 * ps;
-    double pc = 1.0 / 87178291200.0;
-    pc = -1.0 / 479001600.0 + r2 * pc;
-    pc = 1.0 / 3628800.0 + r2 * pc;
-    pc = -1.0 / 40320.0 + r2 * pc;
-    pc = 1.0 / 720.0 + r2 * pc;
-    pc = -1.0 / 24.0 + r2 * pc;
-    pc = 0.5 - r2 * pc;
-    const double cr = 1.0 - r2 * pc;
+    // sin r = r + r^3 P(r^2), cos r = 1 + r^2 Q(r^2): Taylor coefficients to r^15 / r^14
+    double ps = -1.0 / 1307674368000.0;
+    ps = 1.0 / 6227020800.0 + r2 * ps;
+    ps = -1.0 / 39916800.0 + r2 * ps;
+    ps = 1.0 / 362880.0 + r2 * ps;
+    ps = -1.0 / 5040.0 + r2 * ps;
+    ps = 1.0 / 120.0 + r2 * ps;
+    ps = -1.0 / 6.0 + r2 * ps;
+    const double sr = r + r * r2 * ps;
+    double pc = -1.0 / 87178291200.0;
+    pc = 1.0 / 479001600.0 + r2 * pc;
+    pc = -1.0 / 3628800.0 + r2 * pc;
+    pc = 1.0 / 40320.0 + r2 * pc;
+    pc = -1.0 / 720.0 + r2 * pc;
+    pc = 1.0 / 24.0 + r2 * pc;
+    pc = -0.5 + r2 * pc;
+    const double cr = 1.0 + r2 * pc;
     const int qq = (int)((long long)q & 3);
     if (qq == 0) { *s = sr; *c = cr; }
     else if (qq == 1) { *s = cr; *c = -sr; }

@@ -143,6 +143,7 @@ struct gpr_scene {
     // device copies
     gpr::DevBuf d_pos, d_nrm, d_flags, d_mat, d_mats, d_lights, d_stats;
     gpr::BvhDev bvh;
+    bool flattened = false;
     bool committed = false;
     float r2c[16];
     uint64_t last_nv = 0, last_nt = 0;

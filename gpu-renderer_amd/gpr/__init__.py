@@ -89,6 +89,9 @@ _SIG = {
     "gpr_scene_get_info": (C.c_int, [_P, C.POINTER(SceneInfo)]),
     "gpr_scene_export_flat": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "gpr_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "gpr_scene_parse": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "gpr_scene_save": (C.c_int, [_P, C.c_char_p]),
+    "gpr_scene_flatten": (C.c_int, [_P]),
     "gpr_bvh_build": (C.c_int, [_P]),
     "gpr_bvh_export_keys": (C.c_int, [_P, _P, _P, _P]),
     "gpr_bvh_export_ref_layout": (C.c_int, [_P, _P, C.c_size_t]),
@@ -195,9 +198,24 @@ class Scene:
 
     @classmethod
     def load(cls, path):
+        """Load and commit a .xml / .obj / .gprs scene."""
         hh = _P()
         check(lib.gpr_scene_load(os.fsencode(path), C.byref(hh)))
         return cls(hh)
+
+    @classmethod
+    def parse(cls, path):
+        """Load a scene file on the host only (flattened, not committed)."""
+        hh = _P()
+        check(lib.gpr_scene_parse(os.fsencode(path), C.byref(hh)))
+        return cls(hh)
+
+    def save(self, path):
+        check(lib.gpr_scene_save(self.h, os.fsencode(path)))
+
+    def flatten(self):
+        check(lib.gpr_scene_flatten(self.h))
+        return self
 
     def add_mesh(self, p, corners=None, n=None, uv=None, mat=None):
         p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)

@@ -111,6 +111,9 @@ int gpr_scene_set_env(gpr_scene* s, const float rgb[3]);
 /* Flatten meshes, upload, build the light list (scene.cpp:16-20) and the
  * BVH on the device (replaces LBVH_build, src/renderer/bvh.h:186-221). */
 int gpr_scene_commit(gpr_scene* s);
+/* Host half of gpr_scene_commit: flatten the meshes into per-triangle arrays
+ * and collect the emissive triangles (scene.cpp:7-20), without device work. */
+int gpr_scene_flatten(gpr_scene* s);
 int gpr_scene_free(gpr_scene* s);
 
 typedef struct {
@@ -133,8 +136,24 @@ int gpr_scene_export_flat(const gpr_scene* s, float* pos, float* nrm, float* uv,
                           uint32_t* mat, gpr_material* mats);
 
 /* Mitsuba-XML subset / .obj / .gprs scene file -> committed scene
- * (load_scene, src/renderer/loaders/loader.h:11-29). */
+ * (load_scene, src/renderer/loaders/loader.h:11-29; XML semantics of
+ * ParseRecord, src/renderer/loaders/mitsubaloader.cpp:262-522; OBJ corners as
+ * load_obj_file, objloader.cpp:12-121).  The reference's top-level .obj path
+ * is dead code (loader.h:17-20) and it has no custom format: here a .obj file
+ * becomes one Lambert mesh (reflectance 0.5) seen by a 40-degree 256x256
+ * camera on +z framing its bounds, and .gprs is the binary format written by
+ * gpr_scene_save. */
 int gpr_scene_load(const char* path, gpr_scene** out);
+/* gpr_scene_load without the commit: parses and assembles the scene on the
+ * host only (no device work), for inspection with gpr_scene_get_info /
+ * gpr_scene_export_flat after a commit, or for conversion with gpr_scene_save. */
+int gpr_scene_parse(const char* path, gpr_scene** out);
+/* Write the meshes, materials, camera, film size and constant environment of
+ * a scene to a .gprs file (little-endian: "GPRS", u32 version 1, u32 meshes,
+ * u32 width, u32 height, f32 fov, f32 c2w[16], u32 has_env, f32 env[3]; per
+ * mesh: gpr_material, u32 np, nn, nuv, ntri, f32 p[3np], n[3nn], uv[2nuv],
+ * i32 corners[9ntri]). */
+int gpr_scene_save(const gpr_scene* s, const char* path);
 
 /* ---------------------------------------------------------------- BVH --- */
 /* Rebuild the BVH of a committed scene on the device (LBVH_build,

@@ -99,22 +99,23 @@ inline void sincos_core(float x, double* s, double* c) {
     double r = xd - q * kPio2Hi;
     r = r - q * kPio2Lo;
     double r2 = r * r;
-    double ps = 1.0 / 1307674368000.0;
-    ps = -1.0 / 6227020800.0 + r2 * ps;
-    ps = 1.0 / 39916800.0 + r2 * ps;
-    ps = -1.0 / 362880.0 + r2 * ps;
-    ps = 1.0 / 5040.0 + r2 * ps;
-    ps = -1.0 / 120.0 + r2 * ps;
-    ps = 1.0 / 6.0 - r2 * ps;
-    double sr = r - r * r2 * ps;
-    double pc = 1.0 / 87178291200.0;
-    pc = -1.0 / 479001600.0 + r2 * pc;
-    pc = 1.0 / 3628800.0 + r2 * pc;
-    pc = -1.0 / 40320.0 + r2 * pc;
-    pc = 1.0 / 720.0 + r2 * pc;
-    pc = -1.0 / 24.0 + r2 * pc;
-    pc = 0.5 - r2 * pc;
-    double cr = 1.0 - r2 * pc;
+    // sin r = r + r^3 P(r^2), cos r = 1 + r^2 Q(r^2): Taylor coefficients to r^15 / r^14
+    double ps = -1.0 / 1307674368000.0;
+    ps = 1.0 / 6227020800.0 + r2 * ps;
+    ps = -1.0 / 39916800.0 + r2 * ps;
+    ps = 1.0 / 362880.0 + r2 * ps;
+    ps = -1.0 / 5040.0 + r2 * ps;
+    ps = 1.0 / 120.0 + r2 * ps;
+    ps = -1.0 / 6.0 + r2 * ps;
+    double sr = r + r * r2 * ps;
+    double pc = -1.0 / 87178291200.0;
+    pc = 1.0 / 479001600.0 + r2 * pc;
+    pc = -1.0 / 3628800.0 + r2 * pc;
+    pc = 1.0 / 40320.0 + r2 * pc;
+    pc = -1.0 / 720.0 + r2 * pc;
+    pc = 1.0 / 24.0 + r2 * pc;
+    pc = -0.5 + r2 * pc;
+    double cr = 1.0 + r2 * pc;
     int iq = (int)((long long)q & 3);
     switch (iq) {
         case 0: *s = sr; *c = cr; break;
@@ -727,16 +728,20 @@ void rough_diel_sample(const BxDF& b, f3 wo, float u0, float u1, float u2, f3* w
     float phi = kPi * 2 * u1;
     f3 wh = mk(st * ocos(phi), st * osin(phi), ct);
     float dwo = dot(wo, wh);
-    if (dwo == 0) return;
+    // only microfacets facing wo scatter it; a sample whose direction lands on
+    // the wrong side for its lobe has no counterpart in eval/pdf and is dropped
+    if (dwo * wo.z <= 0) return;
     float F = fr_dielectric(dwo, 1.f, b.ior);
     f3 w;
     if (u2 < F) {
         w = reflect(wo, wh);
+        if (w.z * wo.z <= 0) return;
     } else {
         float eta = rd_eta(wo, b.ior);
         f3 n = wh;
         if (dwo < 0) n = -n;
         if (!refract(wo, n, 1.f / eta, &w)) return;
+        if (w.z * wo.z >= 0) return;
     }
     *wi = w;
     rough_diel_eval(b, wo, w, f, pdf);
