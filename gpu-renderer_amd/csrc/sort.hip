@@ -13,12 +13,14 @@
 //            last pass orders the full (key, index).
 //   scatter  4096-key quarters: bucket bases from an LDS scan of the digit
 //            totals, LDS-atomic ranks inside the quarter, 8-byte
-//            (key<<32 | index) composites written into their bucket.
+//            (key<<32 | index) composites staged in LDS in digit order and
+//            written with consecutive lanes on consecutive slots.  The four
+//            quarters of a histogram tile run on one XCD.
 //   local    one block per bucket staged in LDS: MSD split on the next 10 key
-//            bits into 1024 sub-buckets (~1 element each), then an insertion
-//            sort of every sub-bucket on the full composite.  Buckets with a
-//            crowded sub-bucket, or too large for LDS, use LDS bitonic chunk
-//            sorts and pairwise merges.
+//            bits into 1024 sub-buckets (~1 element each), then every
+//            composite is placed by its rank among its sub-bucket.  Buckets
+//            with a crowded sub-bucket, or too large for LDS, use LDS bitonic
+//            chunk sorts and pairwise merges.
 // The digit-total buffer is double-buffered across calls: each call clears
 // the one the next call uses.
 #include "dev_math.h"
@@ -111,53 +113,107 @@ __global__ __launch_bounds__(kTileThreads) void k_hist(const uint32_t* __restric
     }
 }
 
-// ---- scatter: 4096-key sub-tiles, composites into their bucket
+// ---- scatter: 4096-key sub-tiles, composites into their bucket.
+// Workgroups are dispatched round-robin over the 8 XCDs (block b on XCD b%8)
+// and the L2 of one XCD is not coherent with the others', so a cache line
+// written from two XCDs reaches HBM as two partial writes.  The four quarters
+// of one histogram tile own consecutive sub-ranges of every bucket; mapping
+// them onto one XCD lets its L2 merge their writes into whole lines.
+constexpr int kXcds = 8;
+GPR_D uint32_t quarter_of_block(uint32_t b) {
+    const uint32_t g = b / kXcds, x = b % kXcds;
+    const uint32_t t = (g / kQuarters) * kXcds + x;
+    return t * kQuarters + g % kQuarters;
+}
+
 __global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __restrict__ keys, uint32_t n,
                                                           uint32_t shift, uint32_t nd,
                                                           const uint32_t* __restrict__ acc,
                                                           const uint32_t* __restrict__ off,
                                                           uint32_t* __restrict__ bases,
                                                           unsigned long long* __restrict__ tmp) {
-    __shared__ uint32_t pos[kMaxDigits];
+    __shared__ uint32_t gpos[kMaxDigits];  // global slot of digit d's first element, minus its local start
+    __shared__ uint32_t lcnt[kMaxDigits];  // digit counts of the quarter, then local starts
+    __shared__ unsigned long long stage[kSubTile];
     __shared__ uint32_t sh[kTileThreads / 64];
-    const uint64_t b0 = (uint64_t)blockIdx.x * kSubTile;
+    const uint32_t sub = quarter_of_block(blockIdx.x);
+    const uint64_t b0 = (uint64_t)sub * kSubTile;
+    if (b0 >= n) return;  // padding blocks of the XCD mapping
+    // every global load is issued before the first dependent use
     uint32_t k[kSubTile / kTileThreads];
 #pragma unroll
     for (int j = 0; j < kSubTile / kTileThreads; ++j) {
         const uint64_t i = b0 + j * kTileThreads + threadIdx.x;
         k[j] = i < n ? keys[i] : 0u;
     }
-    // bucket bases: exclusive scan of the digit totals (nd/1024 per thread)
     const uint32_t per = nd / kTileThreads;  // nd is a multiple of 1024 or below it
+    uint32_t a0, a1 = 0, o0, o1 = 0;
+    const uint32_t* offs = off + (uint64_t)sub * nd;
     if (per <= 1) {
-        const uint32_t v = threadIdx.x < nd ? acc[threadIdx.x] : 0u;
-        uint32_t tot;
-        const uint32_t ex = scan_incl<kTileThreads>(v, sh, &tot) - v;
-        if (threadIdx.x < nd) pos[threadIdx.x] = ex;
+        a0 = threadIdx.x < nd ? acc[threadIdx.x] : 0u;
+        o0 = threadIdx.x < nd ? offs[threadIdx.x] : 0u;
     } else {
-        uint32_t v[2] = {acc[2 * threadIdx.x], acc[2 * threadIdx.x + 1]};
-        uint32_t tot;
-        const uint32_t ex = scan_incl<kTileThreads>(v[0] + v[1], sh, &tot) - (v[0] + v[1]);
-        pos[2 * threadIdx.x] = ex;
-        pos[2 * threadIdx.x + 1] = ex + v[0];
+        a0 = acc[2 * threadIdx.x];
+        a1 = acc[2 * threadIdx.x + 1];
+        o0 = offs[2 * threadIdx.x];
+        o1 = offs[2 * threadIdx.x + 1];
     }
-    __syncthreads();
-    for (uint32_t d = threadIdx.x; d < nd; d += kTileThreads) {
-        const uint32_t bd = pos[d];
-        if (blockIdx.x == 0) bases[d] = bd;
-        pos[d] = bd + off[(uint64_t)blockIdx.x * nd + d];
-    }
-    __syncthreads();
-    uint32_t slot[kSubTile / kTileThreads];
+    for (uint32_t d = threadIdx.x; d < nd; d += kTileThreads) lcnt[d] = 0;
+    // bucket bases: exclusive scan of the digit totals (its barriers also
+    // order the lcnt clear before the rank atomics)
+    uint32_t tot;
+    const uint32_t ex = scan_incl<kTileThreads>(a0 + a1, sh, &tot) - (a0 + a1);
+    // rank of every key among the quarter's keys of its digit
+    uint32_t r[kSubTile / kTileThreads];
 #pragma unroll
     for (int j = 0; j < kSubTile / kTileThreads; ++j) {
         const uint64_t i = b0 + j * kTileThreads + threadIdx.x;
-        slot[j] = i < n ? atomicAdd(&pos[(k[j] >> shift) & (nd - 1)], 1u) : 0u;
+        r[j] = i < n ? atomicAdd(&lcnt[(k[j] >> shift) & (nd - 1)], 1u) : 0u;
     }
+    __syncthreads();
+    uint32_t l0, l1 = 0;
+    if (per <= 1) {
+        l0 = threadIdx.x < nd ? lcnt[threadIdx.x] : 0u;
+    } else {
+        l0 = lcnt[2 * threadIdx.x];
+        l1 = lcnt[2 * threadIdx.x + 1];
+    }
+    uint32_t ltot;
+    const uint32_t lex = scan_incl<kTileThreads>(l0 + l1, sh, &ltot) - (l0 + l1);
+    if (per <= 1) {
+        if (threadIdx.x < nd) {
+            lcnt[threadIdx.x] = lex;
+            gpos[threadIdx.x] = ex + o0 - lex;
+            if (sub == 0) bases[threadIdx.x] = ex;
+        }
+    } else {
+        lcnt[2 * threadIdx.x] = lex;
+        lcnt[2 * threadIdx.x + 1] = lex + l0;
+        gpos[2 * threadIdx.x] = ex + o0 - lex;
+        gpos[2 * threadIdx.x + 1] = ex + a0 + o1 - (lex + l0);
+        if (sub == 0) {
+            bases[2 * threadIdx.x] = ex;
+            bases[2 * threadIdx.x + 1] = ex + a0;
+        }
+    }
+    __syncthreads();
+    // stage the composites in digit order, then write them with consecutive
+    // lanes on consecutive slots of each digit's run
 #pragma unroll
     for (int j = 0; j < kSubTile / kTileThreads; ++j) {
         const uint64_t i = b0 + j * kTileThreads + threadIdx.x;
-        if (i < n) tmp[slot[j]] = ((unsigned long long)k[j] << 32) | (unsigned long long)i;
+        if (i < n)
+            stage[lcnt[(k[j] >> shift) & (nd - 1)] + r[j]] = ((unsigned long long)k[j] << 32) | (unsigned long long)i;
+    }
+    __syncthreads();
+    const uint32_t m = (uint32_t)min<uint64_t>(kSubTile, n - b0);
+#pragma unroll
+    for (int j = 0; j < kSubTile / kTileThreads; ++j) {
+        const uint32_t p = j * kTileThreads + threadIdx.x;
+        if (p < m) {
+            const unsigned long long w = stage[p];
+            tmp[gpos[(uint32_t)(w >> (32 + shift)) & (nd - 1)] + p] = w;
+        }
     }
 }
 
@@ -188,7 +244,7 @@ GPR_D uint32_t lower_bound_u64(const unsigned long long* a, uint32_t n, unsigned
     return lo;
 }
 
-// ---- local: one bucket per block, LDS MSD split + insertion sort
+// ---- local: one bucket per block, LDS MSD split + rank inside each sub-bucket
 __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const uint32_t* __restrict__ acc,
                                                          const uint32_t* __restrict__ bases,
                                                          unsigned long long* __restrict__ tmp,
@@ -197,12 +253,13 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const u
                                                          uint32_t* __restrict__ sidx) {
     __shared__ unsigned long long buf[2 * kFastCap];  // A | B
     __shared__ uint32_t cnt[kSub];
+    __shared__ uint32_t start[kSub];
     __shared__ uint32_t sh[kLocalThreads / 64];
     __shared__ uint32_t crowded;
     const uint32_t d = blockIdx.x;
     const uint32_t n = acc[d];
-    if (n == 0) return;
     const uint32_t b0 = bases[d];
+    if (n == 0) return;
     unsigned long long* src = tmp + b0;
     const uint32_t sub_bits = min(10u, shift);
     const uint32_t nsub = 1u << sub_bits;
@@ -210,13 +267,23 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const u
     if (n <= (uint32_t)kFastCap) {
         unsigned long long* A = buf;
         unsigned long long* B = buf + kFastCap;
+        // all loads in flight before the first LDS use
+        constexpr int kPer = (kFastCap + kLocalThreads - 1) / kLocalThreads;
+        unsigned long long v[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t i = j * kLocalThreads + threadIdx.x;
+            v[j] = i < n ? src[i] : 0ull;
+        }
         for (uint32_t s = threadIdx.x; s < kSub; s += kLocalThreads) cnt[s] = 0;
         if (threadIdx.x == 0) crowded = 0;
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += kLocalThreads) {
-            const unsigned long long v = src[i];
-            A[i] = v;
-            atomicAdd(&cnt[(uint32_t)(v >> sub_shift) & (nsub - 1)], 1u);
+        uint32_t r[kPer], sb[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t i = j * kLocalThreads + threadIdx.x;
+            sb[j] = (uint32_t)(v[j] >> sub_shift) & (nsub - 1);
+            r[j] = i < n ? atomicAdd(&cnt[sb[j]], 1u) : 0u;
         }
         __syncthreads();
         uint32_t c4[4], s4 = 0;
@@ -230,34 +297,38 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const u
         uint32_t run = scan_incl<kLocalThreads>(s4, sh, &tot) - s4;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            cnt[4 * threadIdx.x + k] = run;
+            start[4 * threadIdx.x + k] = run;
             run += c4[k];
         }
         __syncthreads();
         if (!crowded) {
-            for (uint32_t i = threadIdx.x; i < n; i += kLocalThreads) {
-                const unsigned long long v = A[i];
-                B[atomicAdd(&cnt[(uint32_t)(v >> sub_shift) & (nsub - 1)], 1u)] = v;
+            // sub-bucket order (arrival order inside each sub-bucket) ...
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const uint32_t i = j * kLocalThreads + threadIdx.x;
+                if (i < n) A[start[sb[j]] + r[j]] = v[j];
             }
             __syncthreads();
+            // ... then every composite's rank among its sub-bucket (all distinct)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t e = cnt[4 * threadIdx.x + k], b = e - c4[k];
-                for (uint32_t x = b + 1; x < e; ++x) {
-                    const unsigned long long v = B[x];
-                    uint32_t y = x;
-                    while (y > b && B[y - 1] > v) {
-                        B[y] = B[y - 1];
-                        --y;
-                    }
-                    B[y] = v;
+            for (int j = 0; j < kPer; ++j) {
+                const uint32_t i = j * kLocalThreads + threadIdx.x;
+                if (i < n) {
+                    const uint32_t b = start[sb[j]], c = cnt[sb[j]];
+                    uint32_t rank = 0;
+                    for (uint32_t x = 0; x < c; ++x) rank += A[b + x] < v[j] ? 1u : 0u;
+                    B[b + rank] = v[j];
                 }
             }
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < n; i += kLocalThreads) {
-                const unsigned long long v = B[i];
-                skeys[b0 + i] = (uint32_t)(v >> 32);
-                sidx[b0 + i] = (uint32_t)v;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const uint32_t i = j * kLocalThreads + threadIdx.x;
+                if (i < n) {
+                    const unsigned long long w = B[i];
+                    skeys[b0 + i] = (uint32_t)(w >> 32);
+                    sidx[b0 + i] = (uint32_t)w;
+                }
             }
             return;
         }
@@ -344,7 +415,9 @@ int sort_pairs(const uint32_t* keys, uint32_t* skeys, uint32_t* sidx, uint32_t n
     const uint32_t ntiles = div_up(n, kTile);
     hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kTileThreads), 0, st, keys, n, shift, nd, L.acc[par],
                        L.acc[par ^ 1], L.off);
-    hipLaunchKernelGGL(k_scatter, dim3(div_up(n, kSubTile)), dim3(kTileThreads), 0, st, keys, n, shift, nd,
+    // grid padded to whole groups of kXcds tiles for the XCD mapping
+    const uint32_t nsub_blocks = div_up(ntiles, kXcds) * kXcds * kQuarters;
+    hipLaunchKernelGGL(k_scatter, dim3(nsub_blocks), dim3(kTileThreads), 0, st, keys, n, shift, nd,
                        L.acc[par], L.off, L.bases, L.tmp);
     hipLaunchKernelGGL(k_local, dim3(nd), dim3(kLocalThreads), 0, st, shift, L.acc[par], L.bases, L.tmp,
                        L.alt, skeys, sidx);
