@@ -40,6 +40,16 @@ def parse():
     return ap.parse_args()
 
 
+def pmc_traffic(key):
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes (copy-calibrated), committed under profiles/; None if absent."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic_pmc.json")) as f:
+            return json.load(f)[key]["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def copy_calibration(torch, nbytes=1 << 30, reps=5):
     """Plain device copy of known size: the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters of a profiling run to bytes."""
@@ -210,7 +220,8 @@ def main():
         "config": {"workload": "lbvh_build_ray_query_1M_tris_4M_rays", "triangles": N_TRI,
                    "rays": N_RAYS, "tmax": "inf", "traversal": "bvh4_128B_lds_stack"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": pmc_traffic("k_trace_fast<false>"),
                      "kernel": "k_trace_fast<false>",
                      "bytes_per_ray": round(bytes_per_ray, 2),
                      "mean_nodes": round(n_node, 3), "mean_tris": round(n_tri, 3)},
@@ -219,6 +230,7 @@ def main():
                           "frac": round(sort_bytes / (sort_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "ms_per_sort": round(sort_ms, 5), "ms_per_sort_eager": round(sort_ms_eager, 5),
                           "keys": N_TRI, "bytes_per_key": 68, "iters": iters,
+                          "traffic": pmc_traffic("sort_1M"),
                           "timing": "mean over back-to-back sorts replayed from one hipGraph",
                           "graph_result_ok": sort_ok},
         "anyhit_mrays_s": round(N_RAYS / (ms_any * 1e-3) / 1e6, 2),
