@@ -28,7 +28,10 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
 #include "dev_bsdf.h"
+#include "engine.h"
 #include "traverse.h"
 
 namespace gpr {
@@ -37,39 +40,70 @@ namespace {
 constexpr int kShadeBlock = 256;
 constexpr uint32_t kMaxBounce = 64;
 
-// Queue of path vertices / rays (SoA, capacity = pixel count).
+// Queue of path vertices / rays (SoA, capacity = paths per sample or pass).
 struct QueueView {
     float4* o_tmin;   // origin, tmin
     float4* d_tmax;   // direction, tmax
     float4* hit;      // t, id bits, u, v
     float4* w_pdf;    // throughput rgb, BSDF pdf
-    uint32_t* pix;    // pixel | specular << 31
+    uint32_t* pix;    // frame-buffer pixel | specular << 31
     unsigned long long* rng;  // PCG32 state (pcg mode)
     uint32_t* seed;   // slot-indexed LCG seed (reference mode, persists across samples)
+    uint32_t* job;    // job of the entry (multi-job passes)
 };
 
-// Per-sample control block (zeroed by k_gen).
+// Per-sample control block (counters cleared by k_gen).
 struct Ctrl {
-    uint32_t sample;         // current sample index (advanced by k_finish)
+    uint32_t sample;         // current sample index (single-scene; advanced by k_finish)
     uint32_t error;          // bit0: traversal stack overflow, bit1: look-back timeout
     uint32_t shadow_size[kMaxBounce];
     uint32_t scatter_size[kMaxBounce];
     uint32_t tile_counter[kMaxBounce];
+    uint32_t ext_cursor[kMaxBounce + 1];  // [0] primary, [b+1] scatter queue of bounce b
+    uint32_t sh_cursor[kMaxBounce];
+};
+
+// Jobs of one multi-job pass (device copy of EngineJob[] plus the path offsets).
+struct PassJob {
+    EngineJob j;
+    uint32_t path_off;  // first path of the job in the pass
+    uint32_t pad[3];
+};
+struct PassBuf {
+    uint32_t npaths;
+    uint32_t njobs;
+    uint32_t pad[62];
+    PassJob jobs[kMaxPassJobs];
 };
 
 struct RenderArgs {
-    SceneView sv;
-    QueueView q[3];          // A, B (in/scatter alternate), S (shadow)
+    SceneView sv;              // single-scene renders
+    const SceneView* scenes;   // multi-job passes: scene table
+    const PassBuf* pass;       // multi-job passes: jobs of this pass
+    QueueView q[3];            // A, B (in/scatter alternate), S (shadow)
     float4* fb;
     Ctrl* ctrl;
     unsigned long long* status;  // [kMaxBounce][ntiles] look-back words
     int2* spill;
-    uint32_t npix;
-    uint32_t ntiles;
+    uint32_t npix;             // single-scene: paths per sample
+    uint32_t ntiles;           // look-back words per bounce (capacity / kShadeBlock)
+    uint32_t depth;            // bounces of the sample / pass
+    uint32_t chunk;            // entries a wave claims per atomic in extend / shadow
     uint32_t rng_mode;
     uint32_t rr_start;
     unsigned long long seed;
 };
+
+template <bool MULTI>
+GPR_D uint32_t npaths_of(const RenderArgs& a) {
+    if constexpr (MULTI) return a.pass->npaths;
+    else return a.npix;
+}
+template <bool MULTI>
+GPR_D SceneView scene_of(const RenderArgs& a, const QueueView& q, uint32_t i) {
+    if constexpr (MULTI) return a.scenes[a.pass->jobs[q.job[i]].j.scene];
+    else return a.sv;
+}
 
 GPR_D void fb_add(float4* fb, uint32_t pix, c3 v) {
     float4 x = fb[pix];
@@ -80,15 +114,13 @@ GPR_D void fb_add(float4* fb, uint32_t pix, c3 v) {
 }
 
 // Camera::generateRay (camera.h:86-119), Mitsuba raster mode.
-GPR_D void camera_ray(const SceneView& sv, float px, float py, v3& o, v3& d) {
-    const float* m = sv.r2c;
+GPR_D void camera_ray(const float* m, const float* c, float px, float py, v3& o, v3& d) {
     float x = m[0] * px + m[1] * py + m[2] * 0.f + m[3];
     float y = m[4] * px + m[5] * py + m[6] * 0.f + m[7];
     float z = m[8] * px + m[9] * py + m[10] * 0.f + m[11];
     const float w = m[12] * px + m[13] * py + m[14] * 0.f + m[15];
     v3 pc = V(x, y, z);
     if (!(w == 1)) pc = pc / w;
-    const float* c = sv.c2w;
     x = c[0] * 0.f + c[1] * 0.f + c[2] * 0.f + c[3];
     y = c[4] * 0.f + c[5] * 0.f + c[6] * 0.f + c[7];
     z = c[8] * 0.f + c[9] * 0.f + c[10] * 0.f + c[11];
@@ -100,46 +132,86 @@ GPR_D void camera_ray(const SceneView& sv, float px, float py, v3& o, v3& d) {
                 c[8] * n.x + c[9] * n.y + c[10] * n.z));
 }
 
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_gen(RenderArgs a) {
     const uint32_t tid = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t sample = a.ctrl->sample;
-    // clear the per-sample counters and look-back words
-    const uint32_t nctl = 3 * kMaxBounce;
-    for (uint32_t i = tid; i < nctl; i += gridDim.x * 256) a.ctrl->shadow_size[i] = 0u;
-    const uint64_t nstat = (uint64_t)kMaxBounce * a.ntiles;
-    for (uint64_t i = tid; i < nstat; i += (uint64_t)gridDim.x * 256) a.status[i] = 0ull;
-    if (tid >= a.npix) return;
-    const uint32_t W = a.sv.width;
-    const uint32_t x = tid % W, y = tid / W;
-    float jx, jy;
-    const QueueView& q = a.q[0];
-    if (a.rng_mode == GPR_RNG_REFERENCE) {
-        uint32_t s = tea16(tid, sample);  // InitRandom(idx, sample) (pathtracer_kernel.h:92)
-        jx = lcg_next(s);
-        jy = lcg_next(s);
-        q.seed[tid] = s;
-    } else {
-        const uint64_t inc = pcg_inc(sample, tid);
-        uint64_t st = pcg_init(a.seed, inc);
-        jx = pcg_unit(st, inc);
-        jy = pcg_unit(st, inc);
-        q.rng[tid] = st;
+    const uint32_t gsz = gridDim.x * 256;
+    const uint32_t npix = npaths_of<MULTI>(a);
+    // clear the per-sample counters, cursors and the look-back words this sample uses
+    Ctrl* c = a.ctrl;
+    for (uint32_t i = tid; i < kMaxBounce; i += gsz) {
+        c->shadow_size[i] = 0u;
+        c->scatter_size[i] = 0u;
+        c->tile_counter[i] = 0u;
+        c->ext_cursor[i + 1] = 0u;
+        c->sh_cursor[i] = 0u;
     }
-    v3 o, d;
-    camera_ray(a.sv, (float)x + jx, (float)y + jy, o, d);
-    q.o_tmin[tid] = make_float4(o.x, o.y, o.z, 0.f);
-    q.d_tmax[tid] = make_float4(d.x, d.y, d.z, 1e34f);
-    q.w_pdf[tid] = make_float4(1.f, 1.f, 1.f, 0.f);
-    q.pix[tid] = tid;
+    if (tid == 0) c->ext_cursor[0] = 0u;
+    const uint32_t used_tiles = (npix + kShadeBlock - 1) / kShadeBlock;
+    const uint64_t nstat = (uint64_t)a.depth * used_tiles;
+    for (uint64_t i = tid; i < nstat; i += gsz) a.status[(i / used_tiles) * a.ntiles + i % used_tiles] = 0ull;
+    if (tid >= npix) return;
+    const QueueView& q = a.q[0];
+    if constexpr (MULTI) {
+        // the job holding path tid: last job whose path_off <= tid
+        const PassBuf* pb = a.pass;
+        uint32_t lo = 0, hi = pb->njobs;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pb->jobs[mid].path_off <= tid) lo = mid;
+            else hi = mid;
+        }
+        const EngineJob& jb = pb->jobs[lo].j;
+        const uint32_t lp = tid - pb->jobs[lo].path_off;
+        const uint32_t fbi = jb.fb_off + lp;
+        if (jb.sample == 0) a.fb[fbi] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const uint64_t inc = pcg_inc(jb.sample, lp);
+        uint64_t st = pcg_init(jb.seed, inc);
+        const float jx = pcg_unit(st, inc);
+        const float jy = pcg_unit(st, inc);
+        v3 o, d;
+        camera_ray(jb.r2c, jb.c2w, (float)(lp % jb.width) + jx, (float)(lp / jb.width) + jy, o, d);
+        q.rng[tid] = st;
+        q.o_tmin[tid] = make_float4(o.x, o.y, o.z, 0.f);
+        q.d_tmax[tid] = make_float4(d.x, d.y, d.z, 1e34f);
+        q.w_pdf[tid] = make_float4(1.f, 1.f, 1.f, 0.f);
+        q.pix[tid] = fbi;
+        q.job[tid] = lo;
+    } else {
+        const uint32_t sample = c->sample;
+        const uint32_t W = a.sv.width;
+        const uint32_t x = tid % W, y = tid / W;
+        float jx, jy;
+        if (a.rng_mode == GPR_RNG_REFERENCE) {
+            uint32_t s = tea16(tid, sample);  // InitRandom(idx, sample) (pathtracer_kernel.h:92)
+            jx = lcg_next(s);
+            jy = lcg_next(s);
+            q.seed[tid] = s;
+        } else {
+            const uint64_t inc = pcg_inc(sample, tid);
+            uint64_t st = pcg_init(a.seed, inc);
+            jx = pcg_unit(st, inc);
+            jy = pcg_unit(st, inc);
+            q.rng[tid] = st;
+        }
+        v3 o, d;
+        camera_ray(a.sv.r2c, a.sv.c2w, (float)x + jx, (float)y + jy, o, d);
+        q.o_tmin[tid] = make_float4(o.x, o.y, o.z, 0.f);
+        q.d_tmax[tid] = make_float4(d.x, d.y, d.z, 1e34f);
+        q.w_pdf[tid] = make_float4(1.f, 1.f, 1.f, 0.f);
+        q.pix[tid] = tid;
+    }
 }
 
-GPR_D gpr_hit trace_one(const RenderArgs& a, const Ray3& r, LaneStack& st, bool any, bool ref_order) {
+template <class Stack>
+GPR_D gpr_hit trace_one(const RenderArgs& a, const SceneView& sv, const Ray3& r, Stack& st, bool any,
+                        bool ref_order) {
     gpr_hit h;
     if (ref_order) {
         uint32_t nv = 0, nt = 0;
-        h = any ? ref_trace<true>(a.sv, r, &nv, &nt) : ref_trace<false>(a.sv, r, &nv, &nt);
+        h = any ? ref_trace<true>(sv, r, &nv, &nt) : ref_trace<false>(sv, r, &nv, &nt);
     } else {
-        const bool ok = any ? fast_trace<true>(a.sv, r, st, h) : fast_trace<false>(a.sv, r, st, h);
+        const bool ok = any ? fast_trace<true>(sv, r, st, h) : fast_trace<false>(sv, r, st, h);
         if (!ok) atomicOr(&a.ctrl->error, 1u);
     }
     return h;
@@ -175,26 +247,44 @@ GPR_D float power_heur(float a, float b) {
     return a / (a + b);
 }
 
+// Claim `chunk` consecutive queue entries per wave from a cursor; calls
+// f(i) for every claimed entry below n.
+template <class F>
+GPR_D void for_each_claimed(uint32_t* cursor, uint32_t n, uint32_t chunk, F&& f) {
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(cursor, chunk);
+        base = __shfl(base, 0, 64);
+        if (base >= n) break;
+        for (uint32_t k = 0; k < chunk; k += 64) {
+            const uint32_t i = base + k + lane;
+            if (i < n) f(i);
+        }
+    }
+}
+
 // extend: closest hit; for scattered rays also the BSDF-sampling MIS term
-template <bool PRIMARY, bool REF_ORDER>
-__global__ __launch_bounds__(kTraceBlock) void k_extend(RenderArgs a, uint32_t bounce) {
-    __shared__ int2 lds[kStackLds * kTraceBlock];
-    LaneStack st;
+template <bool PRIMARY, bool REF_ORDER, bool MULTI, int NT, int SL>
+__global__ __launch_bounds__(NT) void k_extend(RenderArgs a, uint32_t bounce) {
+    __shared__ int2 lds[SL * NT];
+    LaneStackT<NT, SL> st;
     st.lds = lds + threadIdx.x;
-    st.spill = a.spill + (uint64_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kStackGlobal;
+    st.spill = a.spill + (uint64_t)(blockIdx.x * NT + threadIdx.x) * kStackGlobal;
     st.sp = 0;
     // queue: primary = A; at bounce b the scatter queue is q[(b+1)&1]
     const QueueView& q = a.q[PRIMARY ? 0 : ((bounce + 1) & 1)];
-    const uint32_t n = PRIMARY ? a.npix : a.ctrl->scatter_size[bounce];
-    const uint32_t stride = gridDim.x * kTraceBlock;
-    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += stride) {
+    const uint32_t n = PRIMARY ? npaths_of<MULTI>(a) : a.ctrl->scatter_size[bounce];
+    uint32_t* cur = &a.ctrl->ext_cursor[PRIMARY ? 0 : bounce + 1];
+    for_each_claimed(cur, n, a.chunk, [&](uint32_t i) {
+        const SceneView sv = scene_of<MULTI>(a, q, i);
         const float4 ot = q.o_tmin[i], dt = q.d_tmax[i];
         Ray3 r;
         r.o = V(ot.x, ot.y, ot.z);
         r.d = V(dt.x, dt.y, dt.z);
         r.tmin = ot.w;
         r.tmax = dt.w;
-        gpr_hit h = trace_one(a, r, st, false, REF_ORDER);
+        gpr_hit h = trace_one(a, sv, r, st, false, REF_ORDER);
         if (!PRIMARY) {
             const float4 wp = q.w_pdf[i];
             const uint32_t pw = q.pix[i];
@@ -202,14 +292,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(RenderArgs a, uint32_t b
             const bool spec = (pw >> 31) != 0;
             const c3 w = C(wp.x, wp.y, wp.z);
             float u_rr = 0.f;
-            if (a.rng_mode == GPR_RNG_REFERENCE) u_rr = lcg_next(q.seed[i]);
+            if (!MULTI && a.rng_mode == GPR_RNG_REFERENCE) u_rr = lcg_next(q.seed[i]);
             if (h.id != -1 && !spec) {
                 const uint32_t t = (uint32_t)h.id;
-                const c3 le = emission_of(a.sv, t);
+                const c3 le = emission_of(sv, t);
                 if (!black(le)) {
-                    const Geom lg = setup_geom(a.sv, t, h.u, h.v);
-                    float lpdf = 1.f / tri_area(a.sv, t);
-                    lpdf /= a.sv.nlights;
+                    const Geom lg = setup_geom(sv, t, h.u, h.v);
+                    float lpdf = 1.f / tri_area(sv, t);
+                    lpdf /= sv.nlights;
                     const float d = h.t;
                     lpdf *= d * d / fabsf(dot3(r.d, lg.ns));
                     const c3 fl = dot3(-r.d, lg.ns) > 0 ? le : C(0.f);
@@ -217,40 +307,40 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(RenderArgs a, uint32_t b
                     fb_add(a.fb, pix, w * fl * wgt);
                 }
             }
-            if (a.rng_mode == GPR_RNG_REFERENCE && cmax(w) < 1 && bounce > 3) {
+            if (!MULTI && a.rng_mode == GPR_RNG_REFERENCE && cmax(w) < 1 && bounce > 3) {
                 // roulette without reweighting; a terminated path is a miss (pathtracer_kernel.h:738-748)
                 const float qq = fmax_h(0.05f, 1 - cmax(w));
                 if (u_rr < qq) h.id = -1;
             }
         }
         q.hit[i] = make_float4(h.t, __int_as_float(h.id), h.u, h.v);
-    }
+    });
 }
 
 // shadow: any hit + accumulate_light_sampling_contribution (pathtracer_kernel.h:642-664)
-template <bool REF_ORDER>
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(RenderArgs a, uint32_t bounce) {
-    __shared__ int2 lds[kStackLds * kTraceBlock];
-    LaneStack st;
+template <bool REF_ORDER, bool MULTI, int NT, int SL>
+__global__ __launch_bounds__(NT) void k_shadow(RenderArgs a, uint32_t bounce) {
+    __shared__ int2 lds[SL * NT];
+    LaneStackT<NT, SL> st;
     st.lds = lds + threadIdx.x;
-    st.spill = a.spill + (uint64_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kStackGlobal;
+    st.spill = a.spill + (uint64_t)(blockIdx.x * NT + threadIdx.x) * kStackGlobal;
     st.sp = 0;
     const QueueView& q = a.q[2];
     const uint32_t n = a.ctrl->shadow_size[bounce];
-    const uint32_t stride = gridDim.x * kTraceBlock;
-    for (uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x; i < n; i += stride) {
+    for_each_claimed(&a.ctrl->sh_cursor[bounce], n, a.chunk, [&](uint32_t i) {
+        const SceneView sv = scene_of<MULTI>(a, q, i);
         const float4 ot = q.o_tmin[i], dt = q.d_tmax[i];
         Ray3 r;
         r.o = V(ot.x, ot.y, ot.z);
         r.d = V(dt.x, dt.y, dt.z);
         r.tmin = ot.w;
         r.tmax = dt.w;
-        const gpr_hit h = trace_one(a, r, st, true, REF_ORDER);
+        const gpr_hit h = trace_one(a, sv, r, st, true, REF_ORDER);
         if (h.id == -1) {
             const float4 wp = q.w_pdf[i];
             fb_add(a.fb, q.pix[i] & 0x7fffffffu, C(wp.x, wp.y, wp.z));
         }
-    }
+    });
 }
 
 // ------------------------------------------------------------- shade ----
@@ -267,22 +357,39 @@ GPR_D unsigned long long pack_status(uint32_t flag, uint32_t sh, uint32_t sc) {
     return ((unsigned long long)flag << 62) | ((unsigned long long)sh << 31) | (unsigned long long)sc;
 }
 
+template <bool MULTI>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bounce) {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[2][kShadeBlock / 64];
     __shared__ uint32_t s_base[2];
-    const uint32_t n_in = bounce == 0 ? a.npix : a.ctrl->scatter_size[bounce - 1];
+    const uint32_t n_in = bounce == 0 ? npaths_of<MULTI>(a) : a.ctrl->scatter_size[bounce - 1];
     if (threadIdx.x == 0) s_tile = atomicAdd(&a.ctrl->tile_counter[bounce], 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t ntiles_in = (n_in + kShadeBlock - 1) / kShadeBlock;
     if (tile >= ntiles_in) return;
     const uint32_t i = tile * kShadeBlock + threadIdx.x;
-    const bool valid = i < n_in;
     const QueueView& qi = a.q[bounce & 1];
     const QueueView& qs = a.q[(bounce + 1) & 1];
     const QueueView& qsh = a.q[2];
-    const SceneView& sv = a.sv;
+    bool valid = i < n_in;
+    uint32_t job = 0, rr_start = a.rr_start, sample = 0, lpix = 0;
+    SceneView sv;
+    if constexpr (MULTI) {
+        if (valid) {
+            job = qi.job[i];
+            const PassJob& pj = a.pass->jobs[job];
+            // a job's paths stop at its own depth
+            valid = bounce < pj.j.max_depth;
+            rr_start = pj.j.rr_start;
+            sample = pj.j.sample;
+            lpix = (qi.pix[i] & 0x7fffffffu) - pj.j.fb_off;
+            sv = a.scenes[pj.j.scene];
+        }
+    } else {
+        sv = a.sv;
+        sample = a.ctrl->sample;
+    }
     bool has_sh = false, has_sc = false;
     Out osh, osc;
     uint32_t pix = 0;
@@ -291,18 +398,19 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
         const float4 dt = qi.d_tmax[i], hh = qi.hit[i], wp = qi.w_pdf[i];
         const uint32_t pw = qi.pix[i];
         pix = pw & 0x7fffffffu;
+        if constexpr (!MULTI) lpix = pix;
         const bool spec = (pw >> 31) != 0;
         const v3 rd = V(dt.x, dt.y, dt.z);
         const c3 w = C(wp.x, wp.y, wp.z);
         const int32_t hid = __float_as_int(hh.y);
         float smp[8];
-        if (a.rng_mode == GPR_RNG_REFERENCE) {
+        if (!MULTI && a.rng_mode == GPR_RNG_REFERENCE) {
             uint32_t s = qi.seed[i];
 #pragma unroll
             for (int k = 0; k < 8; ++k) smp[k] = lcg_next(s);
             qi.seed[i] = s;
         } else {
-            const uint64_t inc = pcg_inc(a.ctrl->sample, pix);
+            const uint64_t inc = pcg_inc(sample, lpix);
             uint64_t s = qi.rng[i];
 #pragma unroll
             for (int k = 0; k < 8; ++k) smp[k] = pcg_unit(s, inc);
@@ -359,7 +467,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
             {
                 const BSample bs = bsdf_sample(m, fr, wo, smp[5], smp[6], smp[4]);
                 c3 ow = bs.pdf == 0.f ? C(0.f) : w * bs.f / bs.pdf;
-                if (a.rng_mode != GPR_RNG_REFERENCE && bounce >= a.rr_start && !black(ow)) {
+                if ((MULTI || a.rng_mode != GPR_RNG_REFERENCE) && bounce >= rr_start && !black(ow)) {
                     const float qq = fmax_h(0.05f, 1 - cmax(ow));
                     if (smp[7] < qq) ow = C(0.f);
                     else ow = ow / (1 - qq);
@@ -442,6 +550,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
         qsh.d_tmax[s] = make_float4(osh.d.x, osh.d.y, osh.d.z, osh.tmax);
         qsh.w_pdf[s] = make_float4(osh.w.r, osh.w.g, osh.w.b, osh.pdf);
         qsh.pix[s] = pix | 0x80000000u;
+        if constexpr (MULTI) qsh.job[s] = job;
     }
     if (has_sc) {
         const uint32_t s = s_base[1] + wsc + lsc;
@@ -450,6 +559,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
         qs.w_pdf[s] = make_float4(osc.w.r, osc.w.g, osc.w.b, osc.pdf);
         qs.pix[s] = pix | (osc.spec ? 0x80000000u : 0u);
         qs.rng[s] = rng_out;
+        if constexpr (MULTI) qs.job[s] = job;
     }
 }
 
@@ -472,27 +582,53 @@ __global__ void k_set_sample(Ctrl* c, uint32_t s) {
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
-}  // namespace gpr
 
-using namespace gpr;
-
-struct gpr_renderer {
-    gpr_scene* scene = nullptr;
-    gpr_render_opts opts{};
-    uint32_t npix = 0, ntiles = 0;
-    uint32_t spp_done = 0;
-    DevBuf fb, ctrl, status, spill;
-    DevBuf qmem[3];
-    DevBuf seeds[2];
-    QueueView q[3];
-    int grid_trace = 0;
-    hipGraphExec_t graph = nullptr;
-    bool graph_ok = false;
+// ---------------------------------------------------------- engine ----
+// Kernel set of one traversal configuration of the multi-job pass.
+namespace {
+struct PassKernels {
+    void (*ext_primary)(RenderArgs, uint32_t);
+    void (*ext_scatter)(RenderArgs, uint32_t);
+    void (*shadow)(RenderArgs, uint32_t);
+    int nt, sl, layout;
 };
+template <int NT, int SL>
+PassKernels pass_kernels() {
+    return {k_extend<true, false, true, NT, SL>, k_extend<false, false, true, NT, SL>,
+            k_shadow<false, true, NT, SL>, NT, SL, 0};
+}
+const PassKernels& pass_kernel_table(int idx) {
+    static const PassKernels t[] = {
+        pass_kernels<64, 8>(),   pass_kernels<64, 16>(),  pass_kernels<64, 32>(),
+        pass_kernels<128, 8>(),  pass_kernels<128, 16>(), pass_kernels<128, 32>(),
+        pass_kernels<256, 8>(),  pass_kernels<256, 16>(), pass_kernels<256, 32>(),
+        pass_kernels<512, 8>(),  pass_kernels<512, 16>(), pass_kernels<512, 32>(),
+        {k_extend<true, true, true, 256, 16>, k_extend<false, true, true, 256, 16>, k_shadow<true, true, 256, 16>,
+         256, 16, 1},
+    };
+    return t[idx];
+}
+constexpr int kPassKernelCount = 13;
+int pass_kernel_index(const TraceCfg& c) {
+    if (c.layout == 1) return 12;
+    const int a = c.nt == 64 ? 0 : c.nt == 128 ? 1 : c.nt == 256 ? 2 : c.nt == 512 ? 3 : -1;
+    const int b = c.sl == 8 ? 0 : c.sl == 16 ? 1 : c.sl == 32 ? 2 : -1;
+    return (a < 0 || b < 0) ? -1 : 3 * a + b;
+}
+}  // namespace
+
+std::vector<TraceCfg> engine_tune_space() {
+    std::vector<TraceCfg> v;
+    for (int nt : {64, 128, 256, 512})
+        for (int sl : {8, 16, 32})
+            for (int chunk : {64, 128, 256, 512}) v.push_back(TraceCfg{nt, sl, chunk, 0});
+    v.push_back(TraceCfg{256, 16, 64, 1});
+    return v;
+}
 
 namespace {
-int alloc_queue(DevBuf& mem, QueueView& q, uint32_t n, DevBuf* seed) {
-    const size_t per = 16 * 4 + 4 + 8;
+int alloc_queue(DevBuf& mem, QueueView& q, uint32_t n, DevBuf* seed, bool with_job) {
+    const size_t per = 16 * 4 + 4 + 8 + (with_job ? 4 : 0);
     GPR_TRY(mem.alloc((size_t)n * per + 8 * 256));
     char* p = mem.as<char>();
     size_t off = 0;
@@ -508,6 +644,7 @@ int alloc_queue(DevBuf& mem, QueueView& q, uint32_t n, DevBuf* seed) {
     q.w_pdf = reinterpret_cast<float4*>(take(16ull * n));
     q.pix = reinterpret_cast<uint32_t*>(take(4ull * n));
     q.rng = reinterpret_cast<unsigned long long*>(take(8ull * n));
+    q.job = with_job ? reinterpret_cast<uint32_t*>(take(4ull * n)) : nullptr;
     q.seed = nullptr;
     if (seed) {
         GPR_TRY(seed->alloc(4ull * n));
@@ -516,7 +653,208 @@ int alloc_queue(DevBuf& mem, QueueView& q, uint32_t n, DevBuf* seed) {
     }
     return GPR_OK;
 }
+}  // namespace
 
+struct Engine {
+    int device = 0;
+    uint32_t capacity = 0, max_jobs = 0, ntiles = 0;
+    DevBuf qmem[3], ctrl, status, spill, pass, scenes;
+    QueueView q[3];
+    int grid[kPassKernelCount] = {};
+    hipStream_t cap_stream = nullptr;
+    static constexpr int kStage = 4;
+    PassBuf* h_pass[kStage] = {};
+    hipEvent_t ev[kStage] = {};
+    int slot = 0;
+    struct GraphKey {
+        uint32_t depth;
+        int kidx;
+        uint32_t chunk;
+        float4* fb;
+        bool operator==(const GraphKey& o) const {
+            return depth == o.depth && kidx == o.kidx && chunk == o.chunk && fb == o.fb;
+        }
+    };
+    std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+    ~Engine() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+        for (int k = 0; k < kStage; ++k) {
+            if (ev[k]) (void)hipEventDestroy(ev[k]);
+            if (h_pass[k]) (void)hipHostFree(h_pass[k]);
+        }
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    }
+};
+
+int engine_create(int device, uint32_t capacity, uint32_t max_jobs, Engine** out) {
+    if (capacity == 0 || capacity >= (1u << 30) || max_jobs == 0 || max_jobs > kMaxPassJobs)
+        return fail(GPR_ERR_INVALID, "engine: capacity or job count out of range");
+    GPR_HIP(hipSetDevice(device));
+    Engine* e = new (std::nothrow) Engine();
+    if (!e) return fail(GPR_ERR_OOM, "engine: host allocation failed");
+    std::unique_ptr<Engine> guard(e);
+    e->device = device;
+    e->capacity = capacity;
+    e->max_jobs = max_jobs;
+    e->ntiles = div_up(capacity, kShadeBlock);
+    for (int k = 0; k < 3; ++k) GPR_TRY(alloc_queue(e->qmem[k], e->q[k], capacity, nullptr, true));
+    GPR_TRY(e->ctrl.alloc(sizeof(Ctrl)));
+    GPR_HIP(hipMemsetAsync(e->ctrl.p, 0, sizeof(Ctrl), stream()));
+    GPR_TRY(e->status.alloc(8ull * kMaxBounce * e->ntiles));
+    GPR_TRY(e->pass.alloc(sizeof(PassBuf)));
+    int cus = 0;
+    GPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    size_t threads = 0;
+    for (int k = 0; k < kPassKernelCount; ++k) {
+        const PassKernels& pk = pass_kernel_table(k);
+        int per_cu = 0;
+        GPR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pk.ext_scatter, pk.nt, 0));
+        // more persistent threads than queue entries would only idle
+        e->grid[k] = std::max(1, std::min<int>(cus * std::max(1, per_cu), (int)div_up(capacity, pk.nt)));
+        threads = std::max(threads, (size_t)e->grid[k] * pk.nt);
+    }
+    GPR_TRY(e->spill.alloc(threads * kStackGlobal * sizeof(int2)));
+    for (int k = 0; k < Engine::kStage; ++k) {
+        GPR_HIP(hipHostMalloc((void**)&e->h_pass[k], sizeof(PassBuf), hipHostMallocDefault));
+        GPR_HIP(hipEventCreateWithFlags(&e->ev[k], hipEventDisableTiming));
+    }
+    GPR_HIP(hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking));
+    GPR_HIP(hipStreamSynchronize(stream()));
+    *out = guard.release();
+    return GPR_OK;
+}
+
+void engine_destroy(Engine* e) { delete e; }
+
+size_t engine_device_bytes(const Engine* e) {
+    return e->qmem[0].bytes + e->qmem[1].bytes + e->qmem[2].bytes + e->ctrl.bytes + e->status.bytes +
+           e->spill.bytes + e->pass.bytes + e->scenes.bytes;
+}
+
+int engine_set_scenes(Engine* e, const std::vector<SceneView>& views, hipStream_t st) {
+    if (views.empty()) return GPR_OK;
+    const size_t bytes = views.size() * sizeof(SceneView);
+    if (bytes > e->scenes.bytes) {
+        GPR_HIP(hipStreamSynchronize(st));
+        GPR_TRY(e->scenes.alloc(bytes));
+        for (auto& g : e->graphs) (void)hipGraphExecDestroy(g.second);
+        e->graphs.clear();
+    }
+    GPR_HIP(hipMemcpyAsync(e->scenes.p, views.data(), bytes, hipMemcpyHostToDevice, st));
+    GPR_HIP(hipStreamSynchronize(st));
+    return GPR_OK;
+}
+
+namespace {
+int record_pass(Engine* e, uint32_t depth, int kidx, uint32_t chunk, float4* fb, hipStream_t st) {
+    RenderArgs a{};
+    a.scenes = e->scenes.as<SceneView>();
+    a.pass = e->pass.as<PassBuf>();
+    for (int k = 0; k < 3; ++k) a.q[k] = e->q[k];
+    a.fb = fb;
+    a.ctrl = e->ctrl.as<Ctrl>();
+    a.status = e->status.as<unsigned long long>();
+    a.spill = e->spill.as<int2>();
+    a.ntiles = e->ntiles;
+    a.depth = depth;
+    a.chunk = chunk;
+    a.rng_mode = GPR_RNG_PCG;
+    const PassKernels& pk = pass_kernel_table(kidx);
+    const dim3 tg(e->grid[kidx]), tb(pk.nt);
+    hipLaunchKernelGGL(k_gen<true>, dim3(div_up(e->capacity, 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(pk.ext_primary, tg, tb, 0, st, a, 0u);
+    for (uint32_t b = 0; b < depth; ++b) {
+        hipLaunchKernelGGL(k_shade<true>, dim3(e->ntiles), dim3(kShadeBlock), 0, st, a, b);
+        hipLaunchKernelGGL(pk.shadow, tg, tb, 0, st, a, b);
+        hipLaunchKernelGGL(pk.ext_scatter, tg, tb, 0, st, a, b);
+    }
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+}  // namespace
+
+int engine_run_pass(Engine* e, const EngineJob* jobs, uint32_t njobs, float4* fb, const TraceCfg& cfg,
+                    hipStream_t st) {
+    if (njobs == 0) return GPR_OK;
+    if (njobs > e->max_jobs) return fail(GPR_ERR_INVALID, "engine: too many jobs in one pass");
+    const int kidx = pass_kernel_index(cfg);
+    if (kidx < 0 || cfg.chunk < 64 || cfg.chunk % 64 != 0 || cfg.chunk > 4096)
+        return fail(GPR_ERR_INVALID, "engine: traversal configuration outside the tuned set");
+    const int s = e->slot;
+    e->slot = (s + 1) % Engine::kStage;
+    GPR_HIP(hipEventSynchronize(e->ev[s]));  // the staging slot's previous upload is done
+    PassBuf* h = e->h_pass[s];
+    uint64_t off = 0;
+    uint32_t depth = 0;
+    for (uint32_t k = 0; k < njobs; ++k) {
+        const EngineJob& j = jobs[k];
+        if (j.npix == 0 || j.width == 0 || j.max_depth > kMaxBounce)
+            return fail(GPR_ERR_INVALID, "engine: malformed job");
+        h->jobs[k].j = j;
+        h->jobs[k].path_off = (uint32_t)off;
+        off += j.npix;
+        depth = std::max(depth, j.max_depth);
+    }
+    if (off > e->capacity) return fail(GPR_ERR_INVALID, "engine: pass exceeds the queue capacity");
+    h->npaths = (uint32_t)off;
+    h->njobs = njobs;
+    const size_t bytes = offsetof(PassBuf, jobs) + sizeof(PassJob) * njobs;
+    GPR_HIP(hipMemcpyAsync(e->pass.p, h, bytes, hipMemcpyHostToDevice, st));
+    GPR_HIP(hipEventRecord(e->ev[s], st));
+    if (depth == 0) depth = 1;
+    const Engine::GraphKey key{depth, kidx, (uint32_t)cfg.chunk, fb};
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : e->graphs)
+        if (g.first == key) exec = g.second;
+    if (!exec) {
+        hipGraph_t graph;
+        GPR_HIP(hipStreamBeginCapture(e->cap_stream, hipStreamCaptureModeThreadLocal));
+        const int rc = record_pass(e, depth, kidx, (uint32_t)cfg.chunk, fb, e->cap_stream);
+        const hipError_t ec = hipStreamEndCapture(e->cap_stream, &graph);
+        GPR_TRY(rc);
+        if (ec != hipSuccess) return fail(GPR_ERR_HIP, std::string("pass capture: ") + hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(GPR_ERR_HIP, std::string("pass instantiate: ") + hipGetErrorString(ei));
+        e->graphs.push_back({key, exec});
+    }
+    GPR_HIP(hipGraphLaunch(exec, st));
+    return GPR_OK;
+}
+
+int engine_errors(Engine* e, hipStream_t st, uint32_t* flags) {
+    uint32_t err = 0;
+    GPR_HIP(hipMemcpyAsync(&err, reinterpret_cast<char*>(e->ctrl.p) + offsetof(Ctrl, error), 4,
+                           hipMemcpyDeviceToHost, st));
+    GPR_HIP(hipStreamSynchronize(st));
+    *flags = err;
+    return GPR_OK;
+}
+
+}  // namespace gpr
+
+using namespace gpr;
+
+struct gpr_renderer {
+    gpr_scene* scene = nullptr;
+    gpr_render_opts opts{};
+    uint32_t npix = 0, ntiles = 0;
+    uint32_t spp_done = 0;
+    DevBuf fb, ctrl, status, spill;
+    DevBuf qmem[3];
+    DevBuf seeds[2];
+    QueueView q[3];
+    int grid_trace = 0;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    SceneView baked{};  // scene view (camera included) the graph was captured with
+    ~gpr_renderer() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    }
+};
+
+namespace {
 RenderArgs make_args(gpr_renderer* r) {
     RenderArgs a{};
     a.sv = r->scene->view();
@@ -527,31 +865,73 @@ RenderArgs make_args(gpr_renderer* r) {
     a.spill = r->spill.as<int2>();
     a.npix = r->npix;
     a.ntiles = r->ntiles;
+    a.depth = r->opts.max_depth;
+    a.chunk = 64;
     a.rng_mode = r->opts.rng_mode;
     a.rr_start = r->opts.rr_start;
     a.seed = r->opts.seed;
     return a;
 }
 
+// One sample: gen -> extend(primary) -> max_depth x {shade, shadow, extend} -> finish.
 int launch_sample(gpr_renderer* r, hipStream_t st) {
     const RenderArgs a = make_args(r);
     const bool ref = r->opts.traversal == GPR_TRAVERSAL_REF;
     const dim3 tg(r->grid_trace), tb(kTraceBlock);
-    hipLaunchKernelGGL(k_gen, dim3(div_up(r->npix, 256)), dim3(256), 0, st, a);
-    if (ref) hipLaunchKernelGGL((k_extend<true, true>), tg, tb, 0, st, a, 0u);
-    else hipLaunchKernelGGL((k_extend<true, false>), tg, tb, 0, st, a, 0u);
+    constexpr int NT = kTraceBlock, SL = kStackLds;
+    hipLaunchKernelGGL(k_gen<false>, dim3(div_up(r->npix, 256)), dim3(256), 0, st, a);
+    if (ref) hipLaunchKernelGGL((k_extend<true, true, false, NT, SL>), tg, tb, 0, st, a, 0u);
+    else hipLaunchKernelGGL((k_extend<true, false, false, NT, SL>), tg, tb, 0, st, a, 0u);
     for (uint32_t b = 0; b < r->opts.max_depth; ++b) {
-        hipLaunchKernelGGL(k_shade, dim3(r->ntiles), dim3(kShadeBlock), 0, st, a, b);
+        hipLaunchKernelGGL(k_shade<false>, dim3(r->ntiles), dim3(kShadeBlock), 0, st, a, b);
         if (ref) {
-            hipLaunchKernelGGL((k_shadow<true>), tg, tb, 0, st, a, b);
-            hipLaunchKernelGGL((k_extend<false, true>), tg, tb, 0, st, a, b);
+            hipLaunchKernelGGL((k_shadow<true, false, NT, SL>), tg, tb, 0, st, a, b);
+            hipLaunchKernelGGL((k_extend<false, true, false, NT, SL>), tg, tb, 0, st, a, b);
         } else {
-            hipLaunchKernelGGL((k_shadow<false>), tg, tb, 0, st, a, b);
-            hipLaunchKernelGGL((k_extend<false, false>), tg, tb, 0, st, a, b);
+            hipLaunchKernelGGL((k_shadow<false, false, NT, SL>), tg, tb, 0, st, a, b);
+            hipLaunchKernelGGL((k_extend<false, false, false, NT, SL>), tg, tb, 0, st, a, b);
         }
     }
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, a.ctrl);
     GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+
+// The sample sequence captured once into a hipGraph and replayed per sample;
+// captured again when the scene view (camera included) has changed.
+bool same_view(const SceneView& a, const SceneView& b) {
+    return a.ntri == b.ntri && a.nlights == b.nlights && a.pos == b.pos && a.nrm == b.nrm && a.flags == b.flags &&
+           a.mat == b.mat && a.mats == b.mats && a.lights == b.lights && a.ref == b.ref && a.bvh4 == b.bvh4 &&
+           a.tris == b.tris && a.root4 == b.root4 && a.nbvh4 == b.nbvh4 && a.nref == b.nref &&
+           a.has_env == b.has_env && std::memcmp(a.env, b.env, sizeof(a.env)) == 0 &&
+           std::memcmp(a.r2c, b.r2c, sizeof(a.r2c)) == 0 && std::memcmp(a.c2w, b.c2w, sizeof(a.c2w)) == 0 &&
+           a.width == b.width && a.height == b.height;
+}
+
+int sample_graph(gpr_renderer* r, hipGraphExec_t* out) {
+    const SceneView now = r->scene->view();
+    if (r->graph && !same_view(now, r->baked)) {
+        GPR_HIP(hipStreamSynchronize(stream()));
+        (void)hipGraphExecDestroy(r->graph);
+        r->graph = nullptr;
+    }
+    if (!r->graph) {
+        r->baked = now;
+        if (!r->cap_stream) GPR_HIP(hipStreamCreateWithFlags(&r->cap_stream, hipStreamNonBlocking));
+        hipGraph_t g;
+        GPR_HIP(hipStreamBeginCapture(r->cap_stream, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_sample(r, r->cap_stream);
+        const hipError_t ec = hipStreamEndCapture(r->cap_stream, &g);
+        GPR_TRY(rc);
+        if (ec != hipSuccess) return fail(GPR_ERR_HIP, std::string("sample capture: ") + hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&r->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            r->graph = nullptr;
+            return fail(GPR_ERR_HIP, std::string("sample instantiate: ") + hipGetErrorString(ei));
+        }
+    }
+    *out = r->graph;
     return GPR_OK;
 }
 }  // namespace
@@ -578,11 +958,12 @@ int gpr_renderer_create(gpr_scene* s, const gpr_render_opts* o, gpr_renderer** o
     if (rc == GPR_OK) rc = r->ctrl.alloc(sizeof(Ctrl));
     if (rc == GPR_OK) rc = r->status.alloc(8ull * kMaxBounce * r->ntiles);
     for (int k = 0; k < 3 && rc == GPR_OK; ++k)
-        rc = alloc_queue(r->qmem[k], r->q[k], r->npix, k < 2 ? &r->seeds[k] : nullptr);
+        rc = alloc_queue(r->qmem[k], r->q[k], r->npix, k < 2 ? &r->seeds[k] : nullptr, false);
     if (rc == GPR_OK) {
         int cus = 0, per_cu = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend<false, false>, kTraceBlock, 0) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, k_extend<false, false, false, kTraceBlock, kStackLds>, kTraceBlock, 0) != hipSuccess)
             rc = fail(GPR_ERR_HIP, "occupancy query failed");
         r->grid_trace = std::max(1, cus * std::max(1, per_cu));
         r->grid_trace = std::min<int>(r->grid_trace, (int)div_up(npix, kTraceBlock));
@@ -602,6 +983,11 @@ int gpr_renderer_reset(gpr_renderer* r) {
     hipLaunchKernelGGL(k_clear_fb, dim3(div_up(r->npix, 256)), dim3(256), 0, stream(), r->fb.as<float4>(), r->npix);
     hipLaunchKernelGGL(k_set_sample, dim3(1), dim3(64), 0, stream(), r->ctrl.as<Ctrl>(), 0u);
     GPR_HIP(hipGetLastError());
+    if (r->graph) {
+        GPR_HIP(hipStreamSynchronize(stream()));
+        (void)hipGraphExecDestroy(r->graph);
+        r->graph = nullptr;
+    }
     r->spp_done = 0;
     return GPR_OK;
 }
@@ -610,7 +996,9 @@ int gpr_render(gpr_renderer* r, uint32_t spp) {
     if (!r) return fail(GPR_ERR_INVALID, "gpr_render: null renderer");
     if (r->scene->bvh.n == 0) return fail(GPR_ERR_UNSUPPORTED, "gpr_render: scene has no triangles");
     hipStream_t st = stream();
-    for (uint32_t s = 0; s < spp; ++s) GPR_TRY(launch_sample(r, st));
+    hipGraphExec_t g = nullptr;
+    GPR_TRY(sample_graph(r, &g));
+    for (uint32_t s = 0; s < spp; ++s) GPR_HIP(hipGraphLaunch(g, st));
     r->spp_done += spp;
     uint32_t err = 0;
     GPR_HIP(hipMemcpyAsync(&err, reinterpret_cast<char*>(r->ctrl.p) + offsetof(Ctrl, error), 4,

@@ -115,26 +115,32 @@ GPR_D gpr_hit ref_trace(const SceneView& sv, Ray3 r, uint32_t* nvisit, uint32_t*
 }
 
 // ------------------------------------------------------ BVH4 traversal ---
-struct LaneStack {
-    int2* lds;       // this lane's column: entry k at lds[k * kTraceBlock]
-    int2* spill;     // this lane's overflow area (kStackGlobal entries)
+// Per-lane traversal stack: the top SL entries in an LDS column of a block of
+// NT lanes (entry k at lds[k * NT]), older entries spilled to a per-lane global
+// slab of kStackGlobal entries.
+template <int NT, int SL>
+struct LaneStackT {
+    static_assert((SL & (SL - 1)) == 0, "LDS stack depth must be a power of two");
+    int2* lds;    // this lane's column
+    int2* spill;  // this lane's overflow area (kStackGlobal entries)
     int sp;
     GPR_D bool push(int2 e) {
-        if (sp >= kStackLds + kStackGlobal) return false;
-        const int slot = sp & (kStackLds - 1);
-        if (sp >= kStackLds) spill[sp - kStackLds] = lds[slot * kTraceBlock];
-        lds[slot * kTraceBlock] = e;
+        if (sp >= SL + kStackGlobal) return false;
+        const int slot = sp & (SL - 1);
+        if (sp >= SL) spill[sp - SL] = lds[slot * NT];
+        lds[slot * NT] = e;
         ++sp;
         return true;
     }
     GPR_D int2 pop() {
         --sp;
-        const int slot = sp & (kStackLds - 1);
-        const int2 e = lds[slot * kTraceBlock];
-        if (sp >= kStackLds) lds[slot * kTraceBlock] = spill[sp - kStackLds];
+        const int slot = sp & (SL - 1);
+        const int2 e = lds[slot * NT];
+        if (sp >= SL) lds[slot * NT] = spill[sp - SL];
         return e;
     }
 };
+using LaneStack = LaneStackT<kTraceBlock, kStackLds>;
 
 GPR_D void tri_rec_hit(const TriRec* tris, uint32_t ntri, int32_t leaf, const Ray3& r, float& tmax,
                        gpr_hit& hit, bool& found) {
@@ -153,8 +159,8 @@ GPR_D void tri_rec_hit(const TriRec* tris, uint32_t ntri, int32_t leaf, const Ra
 // Closest hit (ANY=false) or any hit (ANY=true) against the BVH4.  Returns
 // false only if the stack capacity was exceeded (the ray is then reported
 // as a miss and the caller raises an error flag).
-template <bool ANY>
-GPR_D bool fast_trace(const SceneView& sv, const Ray3& r, LaneStack& st, gpr_hit& hit) {
+template <bool ANY, class Stack>
+GPR_D bool fast_trace(const SceneView& sv, const Ray3& r, Stack& st, gpr_hit& hit) {
     hit = gpr_hit{INFINITY, -1, 0.f, 0.f};
     float tmax = r.tmax;
     bool found = false;

@@ -52,6 +52,40 @@ class RenderOpts(C.Structure):
                 ("seed", C.c_uint64), ("traversal", C.c_uint32)]
 
 
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_start", C.c_uint32),
+                ("seed", C.c_uint64), ("fov_deg", C.c_float), ("c2w", C.c_float * 16)]
+
+
+class TraceCfg(C.Structure):
+    _fields_ = [("block", C.c_uint32), ("stack", C.c_uint32), ("chunk", C.c_uint32),
+                ("layout", C.c_uint32), ("ms", C.c_float)]
+
+    def __repr__(self):
+        return (f"TraceCfg(block={self.block}, stack={self.stack}, chunk={self.chunk}, "
+                f"layout={self.layout}, ms={self.ms:.3f})")
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("arena_bytes", C.c_uint64), ("capacity", C.c_uint32), ("max_jobs", C.c_uint32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("passes", C.c_uint64), ("paths", C.c_uint64), ("ms", C.c_double)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("arena_bytes", C.c_uint64), ("capacity", C.c_uint32),
+                ("max_jobs", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("passes", C.c_uint64),
+                ("renders_per_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_jobs_per_pass", C.c_double)]
+
+
 class GprError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"gpr error {code}: {msg}")
@@ -106,6 +140,22 @@ _SIG = {
     "gpr_renderer_reset": (C.c_int, [_P]),
     "gpr_renderer_queue_sizes": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "gpr_renderer_free": (C.c_int, [_P]),
+    "gpr_scene_arena_bytes": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "gpr_batch_create": (C.c_int, [C.POINTER(BatchOpts), C.POINTER(_P)]),
+    "gpr_batch_add_scene": (C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
+    "gpr_batch_add_job": (C.c_int, [_P, C.POINTER(Job), C.POINTER(C.c_uint32)]),
+    "gpr_batch_tune": (C.c_int, [_P, C.POINTER(TraceCfg), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "gpr_batch_set_cfg": (C.c_int, [_P, C.POINTER(TraceCfg)]),
+    "gpr_batch_run": (C.c_int, [_P]),
+    "gpr_batch_read": (C.c_int, [_P, C.c_uint32, _P]),
+    "gpr_batch_get_stats": (C.c_int, [_P, C.POINTER(BatchStats)]),
+    "gpr_batch_free": (C.c_int, [_P]),
+    "gpr_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(_P)]),
+    "gpr_server_add_scene": (C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
+    "gpr_server_submit": (C.c_int, [_P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
+    "gpr_server_wait": (C.c_int, [_P, C.c_uint64, _P, C.POINTER(C.c_double)]),
+    "gpr_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "gpr_server_free": (C.c_int, [_P]),
     "gpr_gen_soup": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_float, _P]),
     "gpr_gen_rays": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, _P]),
     "gpr_gen_shadow_rays": (C.c_int, [C.c_uint64, C.c_uint64, _P, C.c_uint64, _P]),
@@ -231,6 +281,10 @@ class Scene:
                                      _ptr(uv), 0 if uv is None else len(uv), _ptr(corners),
                                      len(corners), C.byref(m)))
 
+    def set_camera(self, c2w, fov, w, h):
+        m = np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(16))
+        check(lib.gpr_scene_set_camera(self.h, _ptr(m), fov, w, h))
+
     def set_camera_lookat(self, target, origin, up, fov, w, h):
         t = np.asarray(target, np.float32)
         o = np.asarray(origin, np.float32)
@@ -350,6 +404,135 @@ class Renderer:
     def free(self):
         if self.h:
             lib.gpr_renderer_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def job(scene, width, height, spp, c2w, fov=40.0, max_depth=5, rr_start=3, seed=1):
+    """A render request: scene id, resolution, samples, camera (row-major 4x4 cameraToWorld)."""
+    j = Job()
+    j.scene, j.width, j.height, j.spp = scene, width, height, spp
+    j.max_depth, j.rr_start, j.seed, j.fov_deg = max_depth, rr_start, seed, fov
+    j.c2w[:] = [float(x) for x in np.asarray(c2w, np.float32).ravel()]
+    return j
+
+
+def lookat(target, origin, up=(0, 1, 0)):
+    """cameraToWorld of LookAt (row-major), as gpr_scene_set_camera_lookat builds it."""
+    t, o, u = (np.asarray(v, np.float64) for v in (target, origin, up))
+    fwd = (t - o) / np.linalg.norm(t - o)
+    left = np.cross(u, fwd)
+    left /= np.linalg.norm(left)
+    rup = np.cross(fwd, left)
+    m = np.eye(4, dtype=np.float64)
+    m[:3, 0], m[:3, 1], m[:3, 2], m[:3, 3] = left, rup, fwd, o
+    return m.astype(np.float32)
+
+
+def arena_bytes(scene):
+    b = C.c_uint64()
+    check(lib.gpr_scene_arena_bytes(scene.h, C.byref(b)))
+    return b.value
+
+
+class Batch:
+    """Many jobs over many scenes in one multi-job wavefront (gpr_batch_*)."""
+
+    def __init__(self, arena_bytes, capacity=1 << 22, max_jobs=256):
+        o = BatchOpts(arena_bytes, capacity, max_jobs)
+        h = _P()
+        check(lib.gpr_batch_create(C.byref(o), C.byref(h)))
+        self.h = h
+        self.jobs = []
+
+    def add_scene(self, scene):
+        i = C.c_uint32()
+        check(lib.gpr_batch_add_scene(self.h, scene.h, C.byref(i)))
+        return i.value
+
+    def add_job(self, j):
+        i = C.c_uint32()
+        check(lib.gpr_batch_add_job(self.h, C.byref(j), C.byref(i)))
+        self.jobs.append((j.width, j.height))
+        return i.value
+
+    def tune(self, cap=64):
+        out = (TraceCfg * cap)()
+        n = C.c_uint32()
+        check(lib.gpr_batch_tune(self.h, out, cap, C.byref(n)))
+        return [out[k] for k in range(n.value)]
+
+    def set_cfg(self, block, stack, chunk, layout=0):
+        c = TraceCfg(block, stack, chunk, layout, 0.0)
+        check(lib.gpr_batch_set_cfg(self.h, C.byref(c)))
+
+    def run(self):
+        check(lib.gpr_batch_run(self.h))
+
+    def read(self, job):
+        w, h = self.jobs[job]
+        rgb = np.empty((w * h, 3), np.float32)
+        check(lib.gpr_batch_read(self.h, job, _ptr(rgb)))
+        return rgb
+
+    def stats(self):
+        st = BatchStats()
+        check(lib.gpr_batch_get_stats(self.h, C.byref(st)))
+        return st
+
+    def free(self):
+        if self.h:
+            lib.gpr_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request server with HBM-budget admission and merged per-bounce queues (gpr_server_*)."""
+
+    def __init__(self, hbm_budget, arena_bytes, capacity=1 << 22, max_jobs=256):
+        o = ServerOpts(hbm_budget, arena_bytes, capacity, max_jobs)
+        h = _P()
+        check(lib.gpr_server_create(C.byref(o), C.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def add_scene(self, scene):
+        i = C.c_uint32()
+        check(lib.gpr_server_add_scene(self.h, scene.h, C.byref(i)))
+        return i.value
+
+    def submit(self, j):
+        t = C.c_uint64()
+        check(lib.gpr_server_submit(self.h, C.byref(j), C.byref(t)))
+        self.sizes[t.value] = j.width * j.height
+        return t.value
+
+    def wait(self, ticket, want_image=True):
+        n = self.sizes.pop(ticket)
+        rgb = np.empty((n, 3), np.float32) if want_image else None
+        lat = C.c_double()
+        check(lib.gpr_server_wait(self.h, ticket, _ptr(rgb), C.byref(lat)))
+        return rgb, lat.value
+
+    def stats(self):
+        st = ServerStats()
+        check(lib.gpr_server_get_stats(self.h, C.byref(st)))
+        return st
+
+    def free(self):
+        if self.h:
+            lib.gpr_server_free(self.h)
             self.h = None
 
     def __del__(self):

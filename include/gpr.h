@@ -213,6 +213,95 @@ int gpr_renderer_reset(gpr_renderer* r);
 int gpr_renderer_queue_sizes(gpr_renderer* r, uint32_t* shadow, uint32_t* scatter, uint32_t cap);
 int gpr_renderer_free(gpr_renderer* r);
 
+/* ------------------------------------------------- batched and served --- */
+/* Both forms render many jobs (scene, camera, resolution, spp) through one
+ * multi-job wavefront: scenes are copied into one device arena, every pass
+ * renders the next sample of each job it holds with all their paths in shared
+ * ray queues, and the pass is replayed from a hipGraph.  They run in PCG mode
+ * with BVH4 traversal, and a job's image is bit-identical to the single-scene
+ * renderer's (gpr_renderer_create with GPR_RNG_PCG, GPR_TRAVERSAL_FAST, the
+ * same seed / depth / roulette start, the same camera) whatever other jobs it
+ * shares passes with.  No counterpart in the reference: these forms are new
+ * components built around its path. */
+typedef struct {
+    uint32_t scene;     /* scene id returned by gpr_batch_add_scene / gpr_server_add_scene */
+    uint32_t width, height;
+    uint32_t spp;
+    uint32_t max_depth; /* 1..64 */
+    uint32_t rr_start;  /* roulette from this bounce on (unbiased) */
+    uint64_t seed;      /* PCG seed */
+    float fov_deg;      /* Mitsuba perspective, as gpr_scene_set_camera */
+    float c2w[16];      /* row-major cameraToWorld */
+} gpr_job;
+
+/* Traversal launch configuration picked by the autotuner. */
+typedef struct {
+    uint32_t block;     /* threads per block of the extend / shadow kernels */
+    uint32_t stack;     /* LDS stack entries per lane */
+    uint32_t chunk;     /* queue entries claimed per wave per atomic */
+    uint32_t layout;    /* 0 BVH4 128-B nodes, 1 reference DFS layout */
+    float ms;           /* time of the probe pass with this configuration */
+} gpr_trace_cfg;
+
+typedef struct gpr_batch gpr_batch;
+typedef struct {
+    uint64_t arena_bytes; /* device bytes for the scene arena (see gpr_scene_arena_bytes) */
+    uint32_t capacity;    /* paths per pass (queue entries); >= the largest job's pixel count */
+    uint32_t max_jobs;    /* jobs per pass, <= 1024 */
+} gpr_batch_opts;
+/* Device bytes a committed scene takes in an arena. */
+int gpr_scene_arena_bytes(const gpr_scene* s, uint64_t* bytes);
+int gpr_batch_create(const gpr_batch_opts* opts, gpr_batch** out);
+/* Copy a committed scene into the arena (the scene may be freed afterwards). */
+int gpr_batch_add_scene(gpr_batch* b, const gpr_scene* s, uint32_t* id);
+int gpr_batch_add_job(gpr_batch* b, const gpr_job* job, uint32_t* id);
+/* Autotune the traversal configuration on the batch's own first pass (block
+ * size {64,128,256,512} x LDS stack {8,16,32} x chunk {64..512}, plus the
+ * reference layout); best first in out[0..n), n <= cap. */
+int gpr_batch_tune(gpr_batch* b, gpr_trace_cfg* out, uint32_t cap, uint32_t* n);
+int gpr_batch_set_cfg(gpr_batch* b, const gpr_trace_cfg* cfg);
+/* Render every job to completion (synchronous). */
+int gpr_batch_run(gpr_batch* b);
+/* Linear radiance sum of a job (width*height*3 floats) after gpr_batch_run. */
+int gpr_batch_read(gpr_batch* b, uint32_t job, float* rgb_sum);
+typedef struct {
+    uint64_t passes;
+    uint64_t paths;     /* camera paths rendered */
+    double ms;          /* wall time of the last gpr_batch_run */
+} gpr_batch_stats;
+int gpr_batch_get_stats(const gpr_batch* b, gpr_batch_stats* out);
+int gpr_batch_free(gpr_batch* b);
+
+typedef struct gpr_server gpr_server;
+typedef struct {
+    uint64_t hbm_budget;  /* device bytes for arena + queues + frame buffers */
+    uint64_t arena_bytes; /* part of the budget for scenes */
+    uint32_t capacity;    /* paths per pass */
+    uint32_t max_jobs;    /* jobs in flight, <= 1024 */
+} gpr_server_opts;
+/* Starts the server's loop thread.  Frame buffers of admitted requests come
+ * out of hbm_budget - arena_bytes - the engine's queues; a request waits until
+ * its frame buffer fits and a job slot is free. */
+int gpr_server_create(const gpr_server_opts* opts, gpr_server** out);
+int gpr_server_add_scene(gpr_server* s, const gpr_scene* scene, uint32_t* id);
+/* Queue a request; returns at once with a ticket.  GPR_ERR_BUDGET if the
+ * request alone can never fit. */
+int gpr_server_submit(gpr_server* s, const gpr_job* job, uint64_t* ticket);
+/* Block until the request is done; copy its linear radiance sum
+ * (width*height*3 floats, may be NULL) and its latency (submit -> result
+ * readable, ms, may be NULL).  Each ticket is waited once. */
+int gpr_server_wait(gpr_server* s, uint64_t ticket, float* rgb_sum, double* latency_ms);
+typedef struct {
+    uint64_t submitted, completed;
+    uint64_t passes;
+    double renders_per_s;  /* completed / (last completion - first submit) */
+    double p50_ms, p99_ms; /* latency percentiles of the completed requests */
+    double mean_jobs_per_pass;
+} gpr_server_stats;
+int gpr_server_get_stats(gpr_server* s, gpr_server_stats* out);
+/* Stops the loop after the requests in flight are done, frees everything. */
+int gpr_server_free(gpr_server* s);
+
 /* --------------------------------------------------- synthetic inputs --- */
 /* Seeded generators of the benchmark workloads (PCG32 streams). */
 int gpr_gen_soup(uint64_t seed, uint64_t stream, uint32_t n, float sigma, float* pos);
