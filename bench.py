@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gpu-renderer_amd"))
 
@@ -37,6 +39,15 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays in the CPU baseline sample")
     ap.add_argument("--only", choices=["all", "trace", "sort", "copy"], default="all",
                     help="profiling runs: restrict the GPU work to one kernel family")
+    ap.add_argument("--workload", choices=["raycast", "render", "batch", "serve"], default="raycast",
+                    help="raycast: 1M-triangle LBVH + ray query (the headline metric, with a served-stream "
+                         "segment); render: 1920x1080 sphere-grid path tracer; batch: 256 scenes in one "
+                         "run; serve: the Poisson request stream")
+    ap.add_argument("--requests", type=int, default=0,
+                    help="served requests (default: 48 in the raycast run, 256 with --workload serve)")
+    ap.add_argument("--scenes", type=int, default=0, help="scenes of the batch / serve workloads (default 256 / 64)")
+    ap.add_argument("--no-serve", action="store_true", help="raycast run without the served-stream segment")
+    ap.add_argument("--capacity", type=int, default=1 << 23, help="paths per pass of the served / batched engine")
     return ap.parse_args()
 
 
@@ -78,6 +89,222 @@ def cpu_baseline(pos, rays_np, sample):
                       f"traversal of the CPU restatement, {cores} threads, {dt:.2f} s"}
 
 
+# ------------------------------------------------------------------ render --
+def run_render(a, gpr, np, torch):
+    """Full microfacet path tracer: 8x8 UV spheres (64x32 segments) + ground,
+    two quad lights, 1920x1080, 64 spp, depth 8, roulette from bounce 3, PCG
+    seed 7.  A step is one sample of every pixel."""
+    W, H, SPP = 1920, 1080, 64
+    scene = gpr.Scene.sphere_grid(W, H, 32)
+    info = scene.info()
+    r = gpr.Renderer(scene, max_depth=8, rng_mode=gpr.RNG_PCG, rr_start=3, seed=7,
+                     traversal=gpr.TRAVERSAL_FAST)
+    steps = a.steps if a.steps != 20 else SPP
+    r.render(max(1, a.warmup))
+    r.reset()
+    torch.cuda.synchronize()
+    sh, sc = r.queue_sizes(8)
+    rays_per_sample = W * H + int(sh.sum()) + int(sc.sum())
+    t0 = time.perf_counter()
+    r.render(steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = 1000 * dt / steps
+    out = {"metric": "Mpaths/s, 1920x1080 sphere-grid path tracer", "value": round(W * H / (ms * 1e-3) / 1e6, 2),
+           "unit": "Mpaths/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (procedural sphere grid)",
+           "config": {"workload": "sphere_grid_1920x1080_64spp_depth8", "triangles": info.ntri, "spp": steps,
+                      "max_depth": 8, "rr_start": 3, "seed": 7},
+           "rays_per_sample": rays_per_sample,
+           "mrays_per_s_all_rays": round(rays_per_sample / (ms * 1e-3) / 1e6, 2)}
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_py as O
+        cores = max(1, min(os.cpu_count() or 1, 16))
+        # a bounded sample: 1 spp of a 480x270 crop-equivalent frame of the same scene
+        sw, sh_ = 480, 270
+        small = gpr.Scene.sphere_grid(sw, sh_, 32)
+        f = small.flat()
+        inf = f["info"]
+        osc = O.make_scene(f, np.array(inf.c2w[:], np.float32), inf.fov_deg, sw, sh_)
+        nodes, _, _ = O.lbvh(f["pos"])
+        t0 = time.perf_counter()
+        O.render(osc, nodes, 1, 8, gpr.RNG_PCG, seed=7, rr_start=3, nthreads=cores)
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(sw * sh_ / cdt / 1e6, 4), "unit": "Mpaths/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"1 spp of the same scene at {sw}x{sh_}, depth 8, PCG, CPU restatement "
+                                         f"(depth-first per path, {cores} threads), {cdt:.2f} s"}
+    return out
+
+
+# --------------------------------------------------------- batch / serve --
+def loguniform(rng, lo, hi):
+    return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+
+def orbit_camera(rng, radius):
+    """cameraToWorld on a sphere around the scene centre, looking at it."""
+    z = rng.uniform(-0.2, 1.0)
+    phi = rng.uniform(0, 2 * np.pi)
+    rxy = np.sqrt(max(0.0, 1 - z * z))
+    o = np.array([radius * rxy * np.cos(phi), 0.5 + radius * z, radius * rxy * np.sin(phi)])
+    return o
+
+
+def make_scenes(gpr, n, lo, hi, soup, seed):
+    rng = np.random.default_rng(seed)
+    scenes, tris = [], []
+    for k in range(n):
+        t = int(loguniform(rng, lo, hi))
+        s = gpr.Scene.mix(1000 + k, t, soup)
+        scenes.append(s)
+        tris.append(s.info().ntri)
+    return scenes, tris
+
+
+def run_batch(a, gpr, np_, torch):
+    """256 scenes (sphere grid + random soup, 1e3..2e6 triangles log-uniform),
+    each with its own camera and resolution in {128^2, 256^2, 512^2, 1024^2},
+    16 spp, depth 6, in one autotuned batch."""
+    nsc = a.scenes or 256
+    t0 = time.perf_counter()
+    scenes, tris = make_scenes(gpr, nsc, 1e3, 2e6, 0.5, seed=41)
+    setup_s = time.perf_counter() - t0
+    arena = sum(gpr.arena_bytes(s) for s in scenes)
+    b = gpr.Batch(arena, capacity=a.capacity, max_jobs=256)
+    ids = [b.add_scene(s) for s in scenes]
+    del scenes
+    rng = np.random.default_rng(43)
+    res_choices = [128, 256, 512, 1024]
+    total_px = 0
+    for k in range(nsc):
+        res = int(rng.choice(res_choices))
+        o = orbit_camera(rng, 9.0)
+        j = gpr.job(ids[k], res, res, 16, gpr.lookat((0, 0.5, 0), o), fov=40, max_depth=6, rr_start=3,
+                    seed=100 + k)
+        b.add_job(j)
+        total_px += res * res
+    t1 = time.perf_counter()
+    tuned = b.tune()
+    tune_s = time.perf_counter() - t1
+    torch.cuda.synchronize()
+    b.run()
+    st = b.stats()
+    paths = st.paths
+    out = {"metric": "renders/s, 256-scene batch", "value": round(nsc / (st.ms * 1e-3), 3), "unit": "renders/s",
+           "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(st.ms, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scenes)",
+           "config": {"workload": "batch_256_scenes_16spp_depth6", "scenes": nsc, "triangles_total": int(sum(tris)),
+                      "pixels_total": total_px, "spp": 16, "max_depth": 6},
+           "mpaths_per_s": round(paths / (st.ms * 1e-3) / 1e6, 2), "passes": st.passes,
+           "tuned": {"block": tuned[0].block, "stack": tuned[0].stack, "chunk": tuned[0].chunk,
+                     "layout": tuned[0].layout, "probe_ms": round(tuned[0].ms, 3),
+                     "worst_probe_ms": round(tuned[-1].ms, 3), "configs": len(tuned), "tune_s": round(tune_s, 2)},
+           "setup_s": round(setup_s, 1)}
+    b.free()
+    return out
+
+
+def serve_stream(a, gpr, np_, torch, n_req, n_scenes, verbose=True):
+    """Poisson request stream (lambda = 200/s) over device-resident scenes
+    (sphere grids, 1e4..4e6 triangles log-uniform); each request a random
+    scene, a camera on a sphere around it, resolution in {256^2, 512^2,
+    1024^2} and spp in {4, 16, 64}.  Served with merged queues, then the same
+    requests one at a time through the single-scene renderer."""
+    import threading
+    t0 = time.perf_counter()
+    scenes, tris = make_scenes(gpr, n_scenes, 1e4, 4e6, 0.0, seed=51)
+    setup_s = time.perf_counter() - t0
+    arena = sum(gpr.arena_bytes(s) for s in scenes)
+    budget = arena + (8 << 30)
+    srv = gpr.Server(budget, arena, capacity=a.capacity, max_jobs=256)
+    ids = [srv.add_scene(s) for s in scenes]
+    rng = np.random.default_rng(53)
+    arrivals = np.cumsum(rng.exponential(1 / 200.0, n_req))
+    reqs = []
+    for k in range(n_req):
+        sc = int(rng.integers(0, n_scenes))
+        res = int(rng.choice([256, 512, 1024]))
+        spp = int(rng.choice([4, 16, 64]))
+        o = orbit_camera(rng, 9.0)
+        reqs.append(gpr.job(ids[sc], res, res, spp, gpr.lookat((0, 0.5, 0), o), fov=40, max_depth=8, rr_start=3,
+                            seed=200 + k))
+    # autotune the traversal on the stream's first requests before it starts
+    t_tune = time.perf_counter()
+    tuned = srv.tune(reqs[: min(8, n_req)])
+    tune_s = time.perf_counter() - t_tune
+    tickets = [None] * n_req
+
+    def client():
+        start = time.perf_counter()
+        for k in range(n_req):
+            dt = arrivals[k] - (time.perf_counter() - start)
+            if dt > 0:
+                time.sleep(dt)
+            tickets[k] = srv.submit(reqs[k])
+
+    th = threading.Thread(target=client)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    th.start()
+    served = []
+    for k in range(n_req):
+        while tickets[k] is None:
+            time.sleep(0.0005)
+        served.append(srv.wait(tickets[k], want_image=(k < 4)))
+    th.join()
+    served_s = time.perf_counter() - ts
+    st = srv.stats()
+    srv.free()
+    # the same requests one at a time through the plain tracer
+    tp = time.perf_counter()
+    plain_first = []
+    for k, j in enumerate(reqs):
+        s = scenes[j.scene]
+        s.set_camera(np.array(j.c2w[:], np.float32), j.fov_deg, j.width, j.height)
+        r = gpr.Renderer(s, max_depth=j.max_depth, rng_mode=gpr.RNG_PCG, rr_start=j.rr_start, seed=j.seed,
+                         traversal=gpr.TRAVERSAL_FAST)
+        r.render(j.spp)
+        if k < 4:
+            plain_first.append(r.read()[0])
+        else:
+            torch.cuda.synchronize()
+        r.free()
+    plain_s = time.perf_counter() - tp
+    identical = all(np.array_equal(served[k][0].view(np.uint32), plain_first[k].view(np.uint32))
+                    for k in range(min(4, n_req)))
+    lat = np.array([x[1] for x in served])
+    paths = sum(j.width * j.height * j.spp for j in reqs)
+    return {"renders_per_s": round(n_req / served_s, 3), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_ms": round(float(np.percentile(lat, 99)), 1),
+            "server_renders_per_s": round(st.renders_per_s, 3),
+            "plain_renders_per_s": round(n_req / plain_s, 3),
+            "speedup_vs_plain": round(plain_s / served_s, 3),
+            "requests": n_req, "scenes": n_scenes, "scene_triangles_total": int(sum(tris)),
+            "served_s": round(served_s, 2), "plain_s": round(plain_s, 2),
+            "mpaths_per_s": round(paths / served_s / 1e6, 2),
+            "mean_jobs_per_pass": round(st.mean_jobs_per_pass, 2), "passes": st.passes,
+            "images_identical_to_plain": identical,
+            "tuned": {"block": tuned[0].block, "stack": tuned[0].stack, "chunk": tuned[0].chunk,
+                      "layout": tuned[0].layout, "tune_s": round(tune_s, 2)},
+            "offered_load": "Poisson 200 requests/s",
+            "setup_s": round(setup_s, 1)}
+
+
+def run_serve(a, gpr, np_, torch):
+    n_req = a.requests or 256
+    res = serve_stream(a, gpr, np_, torch, n_req, a.scenes or 64)
+    out = {"metric": "served renders/s (Poisson request stream)", "value": res["renders_per_s"],
+           "unit": "renders/s", "n_gpus": 1, "steps": n_req, "warmup": 0,
+           "ms_per_step": round(1000.0 / max(res["renders_per_s"], 1e-9), 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scenes)",
+           "config": {"workload": "serve_64_scenes_poisson_200_per_s", "requests": n_req},
+           "serving": res}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -93,6 +320,10 @@ def main():
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
     gpr.set_stream(stream.cuda_stream)
+    if a.workload != "raycast":
+        fn = {"render": run_render, "batch": run_batch, "serve": run_serve}[a.workload]
+        print(json.dumps(fn(a, gpr, np, torch)), flush=True)
+        return
     if a.only == "copy":
         copy_calibration(torch)
         return
@@ -243,6 +474,11 @@ def main():
     if not a.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(pos, rays_np, min(a.cpu_sample, N_RAYS))
+    if not a.no_serve:
+        print("[bench] served request stream", file=sys.stderr, flush=True)
+        del scene, d_rays, d_shadow, d_hits
+        torch.cuda.synchronize()
+        out["served"] = serve_stream(a, gpr, np, torch, a.requests or 48, a.scenes or 64)
     print(json.dumps(out), flush=True)
 
 

@@ -189,6 +189,10 @@ struct Pcg {
     }
     float unit() { return float(next() >> 8) * (1.0f / 16777216.0f); }  // [0,1)
     double open01() { return (double(next() >> 8) + 0.5) * (1.0 / 16777216.0); }  // (0,1)
+    float normal() {  // Box-Muller, one value per call
+        const double u1 = open01(), u2 = open01();
+        return (float)(std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+    }
 };
 
 }  // namespace gpr
@@ -707,10 +711,8 @@ static void uv_sphere(MeshBuilder& mb, float cx, float cy, float cz, float radiu
     }
 }
 
-extern "C" int gpr_scene_sphere_grid(uint32_t width, uint32_t height, uint32_t subdiv, gpr_scene** out) {
-    if (subdiv < 2) return fail(GPR_ERR_INVALID, "gpr_scene_sphere_grid: subdiv < 2");
-    gpr_scene* s = nullptr;
-    GPR_TRY(gpr_scene_create(&s));
+// 8x8 UV spheres with the 8 material classes, ground plane and two quad lights.
+static int add_sphere_grid(gpr_scene* s, uint32_t subdiv) {
     // 8 material classes: GGX gold / copper at four roughnesses, smooth and
     // rough glass (IOR 1.5), two Lambert colours.
     const float gold_eta[3] = {0.200438f, 0.924033f, 1.10221f}, gold_k[3] = {3.91295f, 2.45285f, 2.14219f};
@@ -764,8 +766,58 @@ extern "C" int gpr_scene_sphere_grid(uint32_t width, uint32_t height, uint32_t s
     if (rc == GPR_OK) rc = ground.add(s, lw);
     if (rc == GPR_OK) rc = l1.add(s, e1);
     if (rc == GPR_OK) rc = l2.add(s, e2);
+    return rc;
+}
+
+extern "C" int gpr_scene_sphere_grid(uint32_t width, uint32_t height, uint32_t subdiv, gpr_scene** out) {
+    if (subdiv < 2) return fail(GPR_ERR_INVALID, "gpr_scene_sphere_grid: subdiv < 2");
+    gpr_scene* s = nullptr;
+    GPR_TRY(gpr_scene_create(&s));
+    int rc = add_sphere_grid(s, subdiv);
     const float tgt[3] = {0, 0, 0}, org[3] = {0, 6, 10}, up[3] = {0, 1, 0};
     if (rc == GPR_OK) rc = gpr_scene_set_camera_lookat(s, tgt, org, up, 40.f, width, height);
+    if (rc == GPR_OK) rc = gpr_scene_commit(s);
+    if (rc != GPR_OK) {
+        gpr_scene_free(s);
+        return rc;
+    }
+    *out = s;
+    return GPR_OK;
+}
+
+// Sphere grid sized to a triangle budget plus a random triangle soup: spheres
+// take (1 - soup_fraction) of ntri (64 spheres of 4 s (s-1) triangles), the
+// soup the rest, as small Lambert triangles with centres uniform in
+// [-4,4] x [0,2] x [-4,4] and N(0, 0.03^2) vertex offsets (PCG32 seed, stream 7).
+extern "C" int gpr_scene_mix(uint64_t seed, uint32_t ntri, float soup_fraction, gpr_scene** out) {
+    if (!out || ntri < 1000 || !(soup_fraction >= 0.f && soup_fraction <= 1.f))
+        return fail(GPR_ERR_INVALID, "gpr_scene_mix: ntri < 1000 or soup_fraction outside [0, 1]");
+    const double sphere_tris = (1.0 - soup_fraction) * ntri;
+    uint32_t sub = (uint32_t)std::lround(0.5 + std::sqrt(0.25 + sphere_tris / 256.0));
+    if (sub < 2) sub = 2;
+    const uint64_t grid_tris = 256ull * sub * (sub - 1) + 6;
+    const uint32_t soup = grid_tris >= ntri ? 0u : (uint32_t)(ntri - grid_tris);
+    gpr_scene* s = nullptr;
+    GPR_TRY(gpr_scene_create(&s));
+    int rc = add_sphere_grid(s, sub);
+    if (rc == GPR_OK && soup > 0) {
+        Pcg g(seed, 7);
+        std::vector<float> p(9ull * soup);
+        for (uint32_t t = 0; t < soup; ++t) {
+            const float cx = -4.f + 8.f * g.unit(), cy = 2.f * g.unit(), cz = -4.f + 8.f * g.unit();
+            for (int v = 0; v < 3; ++v) {
+                p[9ull * t + 3 * v + 0] = cx + 0.03f * g.normal();
+                p[9ull * t + 3 * v + 1] = cy + 0.03f * g.normal();
+                p[9ull * t + 3 * v + 2] = cz + 0.03f * g.normal();
+            }
+        }
+        std::vector<int32_t> c(9ull * soup, -1);
+        for (uint64_t k = 0; k < 3ull * soup; ++k) c[3 * k] = (int32_t)k;
+        const gpr_material m = lambert(0.6f, 0.6f, 0.6f);
+        rc = gpr_scene_add_mesh(s, p.data(), 3 * soup, nullptr, 0, nullptr, 0, c.data(), soup, &m);
+    }
+    const float tgt[3] = {0, 0, 0}, org[3] = {0, 6, 10}, up[3] = {0, 1, 0};
+    if (rc == GPR_OK) rc = gpr_scene_set_camera_lookat(s, tgt, org, up, 40.f, 256, 256);
     if (rc == GPR_OK) rc = gpr_scene_commit(s);
     if (rc != GPR_OK) {
         gpr_scene_free(s);

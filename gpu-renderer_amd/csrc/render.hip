@@ -670,9 +670,10 @@ struct Engine {
         uint32_t depth;
         int kidx;
         uint32_t chunk;
+        uint32_t size;  // launch size class: paths rounded up to a power of two
         float4* fb;
         bool operator==(const GraphKey& o) const {
-            return depth == o.depth && kidx == o.kidx && chunk == o.chunk && fb == o.fb;
+            return depth == o.depth && kidx == o.kidx && chunk == o.chunk && size == o.size && fb == o.fb;
         }
     };
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
@@ -746,7 +747,9 @@ int engine_set_scenes(Engine* e, const std::vector<SceneView>& views, hipStream_
 }
 
 namespace {
-int record_pass(Engine* e, uint32_t depth, int kidx, uint32_t chunk, float4* fb, hipStream_t st) {
+// Launch grids cover `size` paths (the pass's size class), not the capacity:
+// a small pass launches small grids.
+int record_pass(Engine* e, uint32_t depth, int kidx, uint32_t chunk, uint32_t size, float4* fb, hipStream_t st) {
     RenderArgs a{};
     a.scenes = e->scenes.as<SceneView>();
     a.pass = e->pass.as<PassBuf>();
@@ -760,11 +763,11 @@ int record_pass(Engine* e, uint32_t depth, int kidx, uint32_t chunk, float4* fb,
     a.chunk = chunk;
     a.rng_mode = GPR_RNG_PCG;
     const PassKernels& pk = pass_kernel_table(kidx);
-    const dim3 tg(e->grid[kidx]), tb(pk.nt);
-    hipLaunchKernelGGL(k_gen<true>, dim3(div_up(e->capacity, 256)), dim3(256), 0, st, a);
+    const dim3 tg(std::min<uint32_t>(e->grid[kidx], div_up(size, pk.nt))), tb(pk.nt);
+    hipLaunchKernelGGL(k_gen<true>, dim3(div_up(size, 256)), dim3(256), 0, st, a);
     hipLaunchKernelGGL(pk.ext_primary, tg, tb, 0, st, a, 0u);
     for (uint32_t b = 0; b < depth; ++b) {
-        hipLaunchKernelGGL(k_shade<true>, dim3(e->ntiles), dim3(kShadeBlock), 0, st, a, b);
+        hipLaunchKernelGGL(k_shade<true>, dim3(div_up(size, kShadeBlock)), dim3(kShadeBlock), 0, st, a, b);
         hipLaunchKernelGGL(pk.shadow, tg, tb, 0, st, a, b);
         hipLaunchKernelGGL(pk.ext_scatter, tg, tb, 0, st, a, b);
     }
@@ -802,14 +805,17 @@ int engine_run_pass(Engine* e, const EngineJob* jobs, uint32_t njobs, float4* fb
     GPR_HIP(hipMemcpyAsync(e->pass.p, h, bytes, hipMemcpyHostToDevice, st));
     GPR_HIP(hipEventRecord(e->ev[s], st));
     if (depth == 0) depth = 1;
-    const Engine::GraphKey key{depth, kidx, (uint32_t)cfg.chunk, fb};
+    uint32_t size = 1u << 16;
+    while (size < off) size <<= 1;
+    size = std::min(size, e->capacity);
+    const Engine::GraphKey key{depth, kidx, (uint32_t)cfg.chunk, size, fb};
     hipGraphExec_t exec = nullptr;
     for (auto& g : e->graphs)
         if (g.first == key) exec = g.second;
     if (!exec) {
         hipGraph_t graph;
         GPR_HIP(hipStreamBeginCapture(e->cap_stream, hipStreamCaptureModeThreadLocal));
-        const int rc = record_pass(e, depth, kidx, (uint32_t)cfg.chunk, fb, e->cap_stream);
+        const int rc = record_pass(e, depth, kidx, (uint32_t)cfg.chunk, size, fb, e->cap_stream);
         const hipError_t ec = hipStreamEndCapture(e->cap_stream, &graph);
         GPR_TRY(rc);
         if (ec != hipSuccess) return fail(GPR_ERR_HIP, std::string("pass capture: ") + hipGetErrorString(ec));

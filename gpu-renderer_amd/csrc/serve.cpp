@@ -102,6 +102,43 @@ int make_engine_job(const gpr_job& j, const std::vector<SceneView>& views, uint3
 
 TraceCfg to_cfg(const gpr_trace_cfg& c) { return TraceCfg{(int)c.block, (int)c.stack, (int)c.chunk, (int)c.layout}; }
 
+bool cfg_valid(const TraceCfg& t) {
+    bool ok = false;
+    for (const TraceCfg& s : engine_tune_space()) ok = ok || (s.nt == t.nt && s.sl == t.sl && s.layout == t.layout);
+    return ok && t.chunk >= 64 && t.chunk % 64 == 0 && t.chunk <= 4096;
+}
+
+// Time one probe pass per configuration of the search space (after one
+// untimed pass that captures its graph); results sorted fastest first.
+int tune_engine(Engine* e, const std::vector<EngineJob>& pass, float4* fb, hipStream_t st,
+                std::vector<gpr_trace_cfg>& res) {
+    if (pass.empty()) return fail(GPR_ERR_INVALID, "tune: no probe jobs");
+    hipEvent_t e0, e1;
+    GPR_HIP(hipEventCreate(&e0));
+    GPR_HIP(hipEventCreate(&e1));
+    res.clear();
+    int rc = GPR_OK;
+    for (const TraceCfg& c : engine_tune_space()) {
+        rc = engine_run_pass(e, pass.data(), (uint32_t)pass.size(), fb, c, st);
+        if (rc == GPR_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(GPR_ERR_HIP, "tune: event record");
+        for (int r = 0; r < 2 && rc == GPR_OK; ++r) rc = engine_run_pass(e, pass.data(), (uint32_t)pass.size(), fb, c, st);
+        float ms = 0;
+        if (rc == GPR_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = fail(GPR_ERR_HIP, "tune: event timing");
+        if (rc != GPR_OK) break;
+        res.push_back(gpr_trace_cfg{(uint32_t)c.nt, (uint32_t)c.sl, (uint32_t)c.chunk, (uint32_t)c.layout, ms / 2});
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    GPR_TRY(rc);
+    uint32_t err = 0;
+    GPR_TRY(engine_errors(e, st, &err));
+    if (err) return fail(GPR_ERR_INVALID, "tune: error flags " + std::to_string(err));
+    std::stable_sort(res.begin(), res.end(), [](const gpr_trace_cfg& a, const gpr_trace_cfg& c) { return a.ms < c.ms; });
+    return GPR_OK;
+}
+
 }  // namespace
 }  // namespace gpr
 
@@ -204,11 +241,7 @@ int gpr_batch_add_job(gpr_batch* b, const gpr_job* j, uint32_t* id) {
 int gpr_batch_set_cfg(gpr_batch* b, const gpr_trace_cfg* c) {
     if (!b || !c) return fail(GPR_ERR_INVALID, "gpr_batch_set_cfg: null argument");
     const TraceCfg t = to_cfg(*c);
-    bool ok = false;
-    for (const TraceCfg& s : engine_tune_space())
-        ok = ok || (s.nt == t.nt && s.sl == t.sl && s.layout == t.layout);
-    if (!ok || t.chunk < 64 || t.chunk % 64 || t.chunk > 4096)
-        return fail(GPR_ERR_INVALID, "gpr_batch_set_cfg: configuration outside the tuned set");
+    if (!cfg_valid(t)) return fail(GPR_ERR_INVALID, "gpr_batch_set_cfg: configuration outside the tuned set");
     b->cfg = t;
     return GPR_OK;
 }
@@ -221,34 +254,8 @@ int gpr_batch_tune(gpr_batch* b, gpr_trace_cfg* out, uint32_t cap, uint32_t* n) 
     std::vector<uint32_t> issued(b->jobs.size(), 0);
     std::vector<EngineJob> pass;
     batch_pack(b, issued, pass);
-    hipStream_t st = stream();
-    hipEvent_t e0, e1;
-    GPR_HIP(hipEventCreate(&e0));
-    GPR_HIP(hipEventCreate(&e1));
     std::vector<gpr_trace_cfg> res;
-    for (const TraceCfg& c : engine_tune_space()) {
-        // one untimed pass captures the graph and warms the caches
-        int rc = engine_run_pass(b->eng, pass.data(), (uint32_t)pass.size(), b->fb.as<float4>(), c, st);
-        if (rc == GPR_OK && hipEventRecord(e0, st) != hipSuccess) rc = GPR_ERR_HIP;
-        for (int r = 0; r < 2 && rc == GPR_OK; ++r)
-            rc = engine_run_pass(b->eng, pass.data(), (uint32_t)pass.size(), b->fb.as<float4>(), c, st);
-        float ms = 0;
-        if (rc == GPR_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-            rc = GPR_ERR_HIP;
-        if (rc != GPR_OK) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            return fail(GPR_ERR_HIP, "gpr_batch_tune: probe pass failed");
-        }
-        res.push_back(gpr_trace_cfg{(uint32_t)c.nt, (uint32_t)c.sl, (uint32_t)c.chunk, (uint32_t)c.layout, ms / 2});
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    uint32_t err = 0;
-    GPR_TRY(engine_errors(b->eng, st, &err));
-    if (err) return fail(GPR_ERR_INVALID, "gpr_batch_tune: error flags " + std::to_string(err));
-    std::stable_sort(res.begin(), res.end(), [](const gpr_trace_cfg& a, const gpr_trace_cfg& c) { return a.ms < c.ms; });
+    GPR_TRY(tune_engine(b->eng, pass, b->fb.as<float4>(), stream(), res));
     b->cfg = to_cfg(res[0]);
     if (n) *n = (uint32_t)std::min<size_t>(cap, res.size());
     if (out)
@@ -376,6 +383,7 @@ struct gpr_server {
     uint64_t next_ticket = 1;
     bool stop = false;
     bool scenes_dirty = false;
+    bool in_pass = false;  // the loop is launching a pass outside the lock
     int loop_error = GPR_OK;
     std::string loop_msg;
     std::thread loop;
@@ -461,7 +469,7 @@ void server_loop(gpr_server* s) {
     std::vector<EngineJob> pass;
     std::vector<Request*> pass_reqs;
     while (true) {
-        bool scenes = false;
+        TraceCfg cfg;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
@@ -472,8 +480,15 @@ void server_loop(gpr_server* s) {
                 if (s->stop && s->pending.empty() && s->active.empty()) return;
                 s->cv.wait(lk);
             }
-            scenes = s->scenes_dirty;
-            s->scenes_dirty = false;
+            if (s->scenes_dirty) {
+                if (engine_set_scenes(s->eng, s->arena.views, s->st) != GPR_OK) {
+                    lk.unlock();
+                    loop_fail(s, "server: scene table upload failed");
+                    return;
+                }
+                s->scenes_dirty = false;
+            }
+            cfg = s->cfg;
             // pack: active requests in admission order, one sample each
             pass.clear();
             pass_reqs.clear();
@@ -486,13 +501,16 @@ void server_loop(gpr_server* s) {
                 pass.push_back(j);
                 pass_reqs.push_back(r);
             }
-        }
-        if (scenes && engine_set_scenes(s->eng, s->arena.views, s->st) != GPR_OK) {
-            loop_fail(s, "server: scene table upload failed");
-            return;
+            s->in_pass = !pass.empty();
         }
         if (!pass.empty()) {
-            if (engine_run_pass(s->eng, pass.data(), (uint32_t)pass.size(), s->fb.as<float4>(), s->cfg, s->st) != GPR_OK) {
+            const int prc = engine_run_pass(s->eng, pass.data(), (uint32_t)pass.size(), s->fb.as<float4>(), cfg, s->st);
+            {
+                std::lock_guard<std::mutex> g(s->mu);
+                s->in_pass = false;
+            }
+            s->cv.notify_all();
+            if (prc != GPR_OK) {
                 loop_fail(s, "server: pass launch failed: " + std::string(gpr_last_error()));
                 return;
             }
@@ -600,6 +618,49 @@ int gpr_server_wait(gpr_server* s, uint64_t ticket, float* rgb, double* latency_
     if (rgb) std::memcpy(rgb, r->rgb.data(), sizeof(float) * r->rgb.size());
     if (latency_ms) *latency_ms = r->t_done - r->t_submit;
     s->reqs.erase(it);
+    return GPR_OK;
+}
+
+int gpr_server_set_cfg(gpr_server* s, const gpr_trace_cfg* c) {
+    if (!s || !c) return fail(GPR_ERR_INVALID, "gpr_server_set_cfg: null argument");
+    const TraceCfg t = to_cfg(*c);
+    if (!cfg_valid(t)) return fail(GPR_ERR_INVALID, "gpr_server_set_cfg: configuration outside the tuned set");
+    std::lock_guard<std::mutex> g(s->mu);
+    s->cfg = t;
+    return GPR_OK;
+}
+
+int gpr_server_tune(gpr_server* s, const gpr_job* jobs, uint32_t njobs, gpr_trace_cfg* out, uint32_t cap,
+                    uint32_t* n) {
+    if (!s || !jobs || njobs == 0) return fail(GPR_ERR_INVALID, "gpr_server_tune: no probe jobs");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv.wait(lk, [&] { return !s->in_pass; });
+    if (!s->pending.empty() || !s->active.empty()) return fail(GPR_ERR_INVALID, "gpr_server_tune: server is busy");
+    if (s->scenes_dirty) {
+        GPR_TRY(engine_set_scenes(s->eng, s->arena.views, s->st));
+        s->scenes_dirty = false;
+    }
+    std::vector<EngineJob> pass;
+    uint64_t used = 0;
+    int rc = GPR_OK;
+    for (uint32_t k = 0; k < njobs && rc == GPR_OK; ++k) {
+        EngineJob e;
+        rc = make_engine_job(jobs[k], s->arena.views, s->opts.capacity, &e);
+        if (rc != GPR_OK || used + e.npix > s->opts.capacity || pass.size() >= s->opts.max_jobs) break;
+        uint64_t off = 0;
+        if (!s->pool.take(e.npix, &off)) break;
+        e.fb_off = (uint32_t)off;
+        used += e.npix;
+        pass.push_back(e);
+    }
+    std::vector<gpr_trace_cfg> res;
+    if (rc == GPR_OK) rc = tune_engine(s->eng, pass, s->fb.as<float4>(), s->st, res);
+    for (const EngineJob& e : pass) s->pool.give(e.fb_off, e.npix);
+    GPR_TRY(rc);
+    s->cfg = to_cfg(res[0]);
+    if (n) *n = (uint32_t)std::min<size_t>(cap, res.size());
+    if (out)
+        for (uint32_t k = 0; k < cap && k < res.size(); ++k) out[k] = res[k];
     return GPR_OK;
 }
 

@@ -155,12 +155,16 @@ _SIG = {
     "gpr_server_submit": (C.c_int, [_P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
     "gpr_server_wait": (C.c_int, [_P, C.c_uint64, _P, C.POINTER(C.c_double)]),
     "gpr_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "gpr_server_tune": (C.c_int, [_P, C.POINTER(Job), C.c_uint32, C.POINTER(TraceCfg), C.c_uint32,
+                                  C.POINTER(C.c_uint32)]),
+    "gpr_server_set_cfg": (C.c_int, [_P, C.POINTER(TraceCfg)]),
     "gpr_server_free": (C.c_int, [_P]),
     "gpr_gen_soup": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_float, _P]),
     "gpr_gen_rays": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, _P]),
     "gpr_gen_shadow_rays": (C.c_int, [C.c_uint64, C.c_uint64, _P, C.c_uint64, _P]),
     "gpr_scene_cornell": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     "gpr_scene_sphere_grid": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+    "gpr_scene_mix": (C.c_int, [C.c_uint64, C.c_uint32, C.c_float, C.POINTER(_P)]),
 }
 for _name, (_res, _args) in _SIG.items():
     if hasattr(lib, _name):
@@ -244,6 +248,12 @@ class Scene:
     def sphere_grid(cls, w=1920, h=1080, subdiv=32):
         hh = _P()
         check(lib.gpr_scene_sphere_grid(w, h, subdiv, C.byref(hh)))
+        return cls(hh)
+
+    @classmethod
+    def mix(cls, seed, ntri, soup_fraction=0.0):
+        hh = _P()
+        check(lib.gpr_scene_mix(seed, ntri, soup_fraction, C.byref(hh)))
         return cls(hh)
 
     @classmethod
@@ -511,6 +521,17 @@ class Server:
         i = C.c_uint32()
         check(lib.gpr_server_add_scene(self.h, scene.h, C.byref(i)))
         return i.value
+
+    def tune(self, jobs, cap=64):
+        arr = (Job * len(jobs))(*jobs)
+        out = (TraceCfg * cap)()
+        n = C.c_uint32()
+        check(lib.gpr_server_tune(self.h, arr, len(jobs), out, cap, C.byref(n)))
+        return [out[k] for k in range(n.value)]
+
+    def set_cfg(self, block, stack, chunk, layout=0):
+        c = TraceCfg(block, stack, chunk, layout, 0.0)
+        check(lib.gpr_server_set_cfg(self.h, C.byref(c)))
 
     def submit(self, j):
         t = C.c_uint64()

@@ -291,6 +291,12 @@ int gpr_server_submit(gpr_server* s, const gpr_job* job, uint64_t* ticket);
  * (width*height*3 floats, may be NULL) and its latency (submit -> result
  * readable, ms, may be NULL).  Each ticket is waited once. */
 int gpr_server_wait(gpr_server* s, uint64_t ticket, float* rgb_sum, double* latency_ms);
+/* Autotune the server's traversal configuration on probe jobs (one pass of
+ * their first sample per configuration, as gpr_batch_tune); only while no
+ * request is queued or in flight.  Best first in out[0..n), n <= cap. */
+int gpr_server_tune(gpr_server* s, const gpr_job* jobs, uint32_t njobs, gpr_trace_cfg* out, uint32_t cap,
+                    uint32_t* n);
+int gpr_server_set_cfg(gpr_server* s, const gpr_trace_cfg* cfg);
 typedef struct {
     uint64_t submitted, completed;
     uint64_t passes;
@@ -303,6 +309,10 @@ int gpr_server_get_stats(gpr_server* s, gpr_server_stats* out);
 int gpr_server_free(gpr_server* s);
 
 /* --------------------------------------------------- synthetic inputs --- */
+/* Sphere grid (as gpr_scene_sphere_grid) with its subdivision chosen so the
+ * spheres take (1 - soup_fraction) of ntri triangles, plus a random soup of
+ * small Lambert triangles for the rest (seeded PCG32); committed. */
+int gpr_scene_mix(uint64_t seed, uint32_t ntri, float soup_fraction, gpr_scene** out);
 /* Seeded generators of the benchmark workloads (PCG32 streams). */
 int gpr_gen_soup(uint64_t seed, uint64_t stream, uint32_t n, float sigma, float* pos);
 int gpr_gen_rays(uint64_t seed, uint64_t stream, uint64_t n, gpr_ray* rays);

@@ -114,6 +114,9 @@ def test_server_alone_equals_mix_equals_plain(scenes):
     srv = gpr.Server(budget, arena, capacity=4096, max_jobs=16)
     for s in scenes:
         srv.add_scene(s)
+    res_t = srv.tune(js[:2])
+    assert len(res_t) == 49 and all(res_t[k].ms <= res_t[k + 1].ms for k in range(len(res_t) - 1))
+    srv.set_cfg(64, 16, 128, 0)
     print("[serve] server alone", flush=True)
     alone, _ = srv.wait(srv.submit(js[1]))
     print("[serve] server mix", flush=True)
