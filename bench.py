@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--sort-iters", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="rays in the CPU baseline sample")
     ap.add_argument("--only", choices=["all", "trace", "sort", "copy"], default="all",
                     help="profiling runs: restrict the GPU work to one kernel family")
     ap.add_argument("--workload", choices=["raycast", "render", "batch", "serve"], default="raycast",
@@ -122,8 +122,8 @@ def run_render(a, gpr, np, torch):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_py as O
         cores = max(1, min(os.cpu_count() or 1, 16))
-        # a bounded sample: 1 spp of a 480x270 crop-equivalent frame of the same scene
-        sw, sh_ = 480, 270
+        # a bounded sample: 1 spp of the same scene at 1920x1080
+        sw, sh_ = 1920, 1080
         small = gpr.Scene.sphere_grid(sw, sh_, 32)
         f = small.flat()
         inf = f["info"]
