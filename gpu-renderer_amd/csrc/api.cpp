@@ -483,12 +483,14 @@ int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint
         }
         return GPR_OK;
     }
-    if (flags & GPR_TRACE_DEVICE_PTRS) return launch_trace(sv, rays, hits, n, flags, stats, stream());
+    if (flags & GPR_TRACE_DEVICE_PTRS)
+        return launch_trace(sv, rays, hits, n, flags, stats, stream(), s->trace_nt, s->trace_sl, s->trace_chunk);
     DevBuf dr, dh;
     GPR_TRY(dr.alloc(n * sizeof(gpr_ray)));
     GPR_TRY(dh.alloc(n * sizeof(gpr_hit)));
     GPR_HIP(hipMemcpyAsync(dr.p, rays, n * sizeof(gpr_ray), hipMemcpyHostToDevice, stream()));
-    GPR_TRY(launch_trace(sv, dr.as<gpr_ray>(), dh.as<gpr_hit>(), n, flags, stats, stream()));
+    GPR_TRY(launch_trace(sv, dr.as<gpr_ray>(), dh.as<gpr_hit>(), n, flags, stats, stream(), s->trace_nt,
+                         s->trace_sl, s->trace_chunk));
     GPR_HIP(hipMemcpyAsync(hits, dh.p, n * sizeof(gpr_hit), hipMemcpyDeviceToHost, stream()));
     GPR_HIP(hipStreamSynchronize(stream()));
     unsigned long long ov = 0;
@@ -496,6 +498,61 @@ int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint
         GPR_TRY(trace_overflow_count(&ov));
         if (ov) return fail(GPR_ERR_INVALID, "traversal stack capacity exceeded on " + std::to_string(ov) + " rays");
     }
+    return GPR_OK;
+}
+
+int gpr_trace_set_cfg(gpr_scene* s, const gpr_trace_cfg* c) {
+    if (!s || !c) return fail(GPR_ERR_INVALID, "gpr_trace_set_cfg: null argument");
+    if (c->layout != 0 || !trace_cfg_valid((int)c->block, (int)c->stack, (int)c->chunk))
+        return fail(GPR_ERR_INVALID, "gpr_trace_set_cfg: configuration outside the tuned set");
+    s->trace_nt = (int)c->block;
+    s->trace_sl = (int)c->stack;
+    s->trace_chunk = (int)c->chunk;
+    return GPR_OK;
+}
+
+int gpr_trace_tune(gpr_scene* s, const gpr_ray* rays, uint64_t n, uint32_t flags, gpr_trace_cfg* out, uint32_t cap,
+                   uint32_t* nout) {
+    if (!s || !s->committed || !rays || n == 0) return fail(GPR_ERR_INVALID, "gpr_trace_tune: bad argument");
+    if (!(flags & GPR_TRACE_DEVICE_PTRS) || (flags & GPR_TRACE_REF_ORDER))
+        return fail(GPR_ERR_INVALID, "gpr_trace_tune: needs device rays and the BVH4 traversal");
+    if (s->mat.empty()) return fail(GPR_ERR_INVALID, "gpr_trace_tune: empty scene");
+    const SceneView sv = s->view();
+    DevBuf hits;
+    GPR_TRY(hits.alloc(n * sizeof(gpr_hit)));
+    hipStream_t st = stream();
+    hipEvent_t e0, e1;
+    GPR_HIP(hipEventCreate(&e0));
+    GPR_HIP(hipEventCreate(&e1));
+    std::vector<gpr_trace_cfg> res;
+    int rc = GPR_OK;
+    for (int nt : {64, 128, 256, 512})
+        for (int sl : {8, 16, 32})
+            for (int chunk : {64, 128, 256}) {
+                if (rc != GPR_OK) break;
+                rc = launch_trace(sv, rays, hits.as<gpr_hit>(), n, flags, nullptr, st, nt, sl, chunk);
+                if (rc == GPR_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(GPR_ERR_HIP, "tune: event");
+                for (int r = 0; r < 2 && rc == GPR_OK; ++r)
+                    rc = launch_trace(sv, rays, hits.as<gpr_hit>(), n, flags, nullptr, st, nt, sl, chunk);
+                float ms = 0;
+                if (rc == GPR_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    rc = fail(GPR_ERR_HIP, "tune: event timing");
+                if (rc == GPR_OK) res.push_back(gpr_trace_cfg{(uint32_t)nt, (uint32_t)sl, (uint32_t)chunk, 0, ms / 2});
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    GPR_TRY(rc);
+    unsigned long long ov = 0;
+    GPR_TRY(trace_overflow_count(&ov));
+    if (ov) return fail(GPR_ERR_INVALID, "gpr_trace_tune: traversal stack capacity exceeded");
+    std::stable_sort(res.begin(), res.end(), [](const gpr_trace_cfg& a, const gpr_trace_cfg& b) { return a.ms < b.ms; });
+    s->trace_nt = (int)res[0].block;
+    s->trace_sl = (int)res[0].stack;
+    s->trace_chunk = (int)res[0].chunk;
+    if (nout) *nout = (uint32_t)std::min<size_t>(cap, res.size());
+    if (out)
+        for (uint32_t k = 0; k < cap && k < res.size(); ++k) out[k] = res[k];
     return GPR_OK;
 }
 

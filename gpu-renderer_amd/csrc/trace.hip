@@ -5,6 +5,8 @@
 // global atomic cursor until the batch is drained, so the grid is sized to
 // the device (not to the batch) and the per-lane overflow stacks are indexed
 // by the persistent thread id.
+#include <algorithm>
+
 #include "traverse.h"
 
 namespace gpr {
@@ -28,28 +30,30 @@ GPR_D void store_hit(gpr_hit* hits, uint64_t i, const gpr_hit& h) {
     *reinterpret_cast<float4*>(hits + i) = make_float4(h.t, __int_as_float(h.id), h.u, h.v);
 }
 
-template <bool ANY>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_fast(SceneView sv, const gpr_ray* __restrict__ rays,
-                                                            gpr_hit* __restrict__ hits, uint64_t n,
-                                                            unsigned long long* __restrict__ cursor,
-                                                            int2* __restrict__ spill) {
-    __shared__ int2 lds[kStackLds * kTraceBlock];
-    LaneStack st;
+template <bool ANY, int NT, int SL>
+__global__ __launch_bounds__(NT) void k_trace_fast(SceneView sv, const gpr_ray* __restrict__ rays,
+                                                   gpr_hit* __restrict__ hits, uint64_t n,
+                                                   unsigned long long* __restrict__ cursor,
+                                                   int2* __restrict__ spill, uint32_t chunk) {
+    __shared__ int2 lds[SL * NT];
+    LaneStackT<NT, SL> st;
     st.lds = lds + threadIdx.x;
-    st.spill = spill + (uint64_t)(blockIdx.x * kTraceBlock + threadIdx.x) * kStackGlobal;
+    st.spill = spill + (uint64_t)(blockIdx.x * NT + threadIdx.x) * kStackGlobal;
     st.sp = 0;
     const int lane = threadIdx.x & 63;
     while (true) {
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(cursor, 64ull);
+        if (lane == 0) base = atomicAdd(cursor, (unsigned long long)chunk);
         base = __shfl(base, 0, 64);
         if (base >= n) break;
-        const uint64_t i = base + lane;
-        if (i < n) {
-            const Ray3 r = load_ray(rays, i);
-            gpr_hit h;
-            if (!fast_trace<ANY>(sv, r, st, h)) atomicAdd(cursor + 1, 1ull);  // stack overflow count
-            store_hit(hits, i, h);
+        for (uint32_t k = 0; k < chunk; k += 64) {
+            const uint64_t i = base + k + lane;
+            if (i < n) {
+                const Ray3 r = load_ray(rays, i);
+                gpr_hit h;
+                if (!fast_trace<ANY>(sv, r, st, h)) atomicAdd(cursor + 1, 1ull);  // stack overflow count
+                store_hit(hits, i, h);
+            }
         }
     }
 }
@@ -77,24 +81,55 @@ __global__ __launch_bounds__(256) void k_trace_ref(SceneView sv, const gpr_ray* 
     }
 }
 
+using TraceKernel = void (*)(SceneView, const gpr_ray*, gpr_hit*, uint64_t, unsigned long long*, int2*, uint32_t);
+struct TraceVariant {
+    TraceKernel closest, any;
+    int nt, sl;
+};
+template <int NT, int SL>
+TraceVariant trace_variant() {
+    return {k_trace_fast<false, NT, SL>, k_trace_fast<true, NT, SL>, NT, SL};
+}
+constexpr int kTraceVariants = 12;
+const TraceVariant& trace_variant_at(int k) {
+    static const TraceVariant t[kTraceVariants] = {
+        trace_variant<64, 8>(),  trace_variant<64, 16>(),  trace_variant<64, 32>(),
+        trace_variant<128, 8>(), trace_variant<128, 16>(), trace_variant<128, 32>(),
+        trace_variant<256, 8>(), trace_variant<256, 16>(), trace_variant<256, 32>(),
+        trace_variant<512, 8>(), trace_variant<512, 16>(), trace_variant<512, 32>(),
+    };
+    return t[k];
+}
+int trace_variant_index(int nt, int sl) {
+    const int a = nt == 64 ? 0 : nt == 128 ? 1 : nt == 256 ? 2 : nt == 512 ? 3 : -1;
+    const int b = sl == 8 ? 0 : sl == 16 ? 1 : sl == 32 ? 2 : -1;
+    return (a < 0 || b < 0) ? -1 : 3 * a + b;
+}
+
 struct TraceResources {
     DevBuf spill;
     DevBuf cursor;
-    int grid = 0;
+    int grid[kTraceVariants][2] = {};
     int dev = -1;
 };
 TraceResources g_res[16];
 
 int ensure_resources(TraceResources& R) {
     const int dev = device_id();
-    if (R.dev == dev && R.grid > 0) return GPR_OK;
+    if (R.dev == dev) return GPR_OK;
     int cus = 0;
     GPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    int per_cu = 0;
-    GPR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_fast<false>, kTraceBlock, 0));
-    if (per_cu < 1) per_cu = 1;
-    R.grid = cus * per_cu;
-    GPR_TRY(R.spill.alloc((size_t)R.grid * kTraceBlock * kStackGlobal * sizeof(int2)));
+    size_t threads = 0;
+    for (int k = 0; k < kTraceVariants; ++k) {
+        const TraceVariant& v = trace_variant_at(k);
+        for (int a = 0; a < 2; ++a) {
+            int per_cu = 0;
+            GPR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, a ? v.any : v.closest, v.nt, 0));
+            R.grid[k][a] = cus * std::max(1, per_cu);
+            threads = std::max(threads, (size_t)R.grid[k][a] * v.nt);
+        }
+    }
+    GPR_TRY(R.spill.alloc(threads * kStackGlobal * sizeof(int2)));
     GPR_TRY(R.cursor.alloc(4 * sizeof(unsigned long long)));
     R.dev = dev;
     return GPR_OK;
@@ -102,8 +137,12 @@ int ensure_resources(TraceResources& R) {
 
 }  // namespace
 
+bool trace_cfg_valid(int nt, int sl, int chunk) {
+    return trace_variant_index(nt, sl) >= 0 && chunk >= 64 && chunk % 64 == 0 && chunk <= 4096;
+}
+
 int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags,
-                 unsigned long long* stats, hipStream_t st) {
+                 unsigned long long* stats, hipStream_t st, int nt, int sl, int chunk) {
     if (n == 0) return GPR_OK;
     const bool any = (flags & GPR_TRACE_ANY) != 0;
     if (flags & GPR_TRACE_REF_ORDER) {
@@ -115,16 +154,17 @@ int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64
         GPR_HIP(hipGetLastError());
         return GPR_OK;
     }
+    const int k = trace_variant_index(nt, sl);
+    if (k < 0 || !trace_cfg_valid(nt, sl, chunk)) return fail(GPR_ERR_INVALID, "trace: configuration outside the tuned set");
     TraceResources& R = g_res[device_id() & 15];
     GPR_TRY(ensure_resources(R));
+    const TraceVariant& v = trace_variant_at(k);
     unsigned long long* cur = R.cursor.as<unsigned long long>();
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cur, 4);
-    if (any)
-        hipLaunchKernelGGL(k_trace_fast<true>, dim3(R.grid), dim3(kTraceBlock), 0, st, sv, rays, hits, n,
-                           cur, R.spill.as<int2>());
-    else
-        hipLaunchKernelGGL(k_trace_fast<false>, dim3(R.grid), dim3(kTraceBlock), 0, st, sv, rays, hits, n,
-                           cur, R.spill.as<int2>());
+    const uint64_t want = (n + v.nt - 1) / v.nt;  // more blocks than rays would only idle
+    const dim3 grid((unsigned)std::min<uint64_t>(R.grid[k][any ? 1 : 0], want));
+    hipLaunchKernelGGL(any ? v.any : v.closest, grid, dim3(v.nt), 0, st, sv, rays, hits, n, cur,
+                       R.spill.as<int2>(), (uint32_t)chunk);
     GPR_HIP(hipGetLastError());
     return GPR_OK;
 }
@@ -133,7 +173,7 @@ int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64
 int trace_overflow_count(unsigned long long* out) {
     TraceResources& R = g_res[device_id() & 15];
     *out = 0;
-    if (R.grid == 0) return GPR_OK;
+    if (R.dev < 0) return GPR_OK;
     GPR_HIP(hipMemcpyAsync(out, R.cursor.as<unsigned long long>() + 1, 8, hipMemcpyDeviceToHost, stream()));
     GPR_HIP(hipStreamSynchronize(stream()));
     return GPR_OK;

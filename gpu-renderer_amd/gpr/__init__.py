@@ -132,6 +132,9 @@ _SIG = {
     "gpr_bvh_last_timing": (C.c_int, [_P, _P]),
     "gpr_sort_pairs_device": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "gpr_trace": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32]),
+    "gpr_trace_tune": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.POINTER(TraceCfg), C.c_uint32,
+                                 C.POINTER(C.c_uint32)]),
+    "gpr_trace_set_cfg": (C.c_int, [_P, C.POINTER(TraceCfg)]),
     "gpr_trace_ref_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "gpr_renderer_create": (C.c_int, [_P, C.POINTER(RenderOpts), C.POINTER(_P)]),
     "gpr_render": (C.c_int, [_P, C.c_uint32]),
@@ -360,6 +363,17 @@ class Scene:
     def trace_device(self, rays_ptr, hits_ptr, n, any_hit=False, ref_order=False):
         flags = TRACE_DEVICE_PTRS | (TRACE_ANY if any_hit else 0) | (TRACE_REF_ORDER if ref_order else 0)
         check(lib.gpr_trace(self.h, C.c_void_p(rays_ptr), C.c_void_p(hits_ptr), n, flags))
+
+    def trace_tune(self, rays_ptr, n, any_hit=False, cap=64):
+        out = (TraceCfg * cap)()
+        k = C.c_uint32()
+        fl = TRACE_DEVICE_PTRS | (TRACE_ANY if any_hit else 0)
+        check(lib.gpr_trace_tune(self.h, C.c_void_p(rays_ptr), n, fl, out, cap, C.byref(k)))
+        return [out[i] for i in range(k.value)]
+
+    def trace_set_cfg(self, block, stack, chunk):
+        c = TraceCfg(block, stack, chunk, 0, 0.0)
+        check(lib.gpr_trace_set_cfg(self.h, C.byref(c)))
 
     def ref_stats(self):
         nv, nt = C.c_uint64(), C.c_uint64()

@@ -178,11 +178,29 @@ int gpr_sort_pairs_device(const uint32_t* keys, uint32_t* sorted_keys, uint32_t*
                           uint32_t n);
 
 /* -------------------------------------------------------------- trace --- */
+/* Traversal launch configuration picked by the autotuners (gpr_trace_tune,
+ * gpr_batch_tune, gpr_server_tune). */
+typedef struct {
+    uint32_t block;     /* threads per block of the traversal kernels: 64, 128, 256, 512 */
+    uint32_t stack;     /* LDS stack entries per lane: 8, 16, 32 */
+    uint32_t chunk;     /* rays / queue entries claimed per wave per atomic */
+    uint32_t layout;    /* 0 BVH4 128-B nodes, 1 reference DFS layout (batched / served only) */
+    float ms;           /* time of the probe with this configuration */
+} gpr_trace_cfg;
 #define GPR_TRACE_ANY 1u         /* any-hit: stop at the first accepted hit */
 #define GPR_TRACE_REF_ORDER 2u   /* reference visit order on the DFS layout */
 #define GPR_TRACE_DEVICE_PTRS 4u /* rays / hits are device pointers */
 /* trace / trace_shadow (src/renderer/pathtracer_kernel.h:235-246, 310-321). */
 int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags);
+/* Autotune the BVH4 traversal of gpr_trace for this scene on a batch of device
+ * rays (flags: GPR_TRACE_DEVICE_PTRS, optionally GPR_TRACE_ANY): block size
+ * {64,128,256,512} x LDS stack {8,16,32} x chunk {64,128,256}; the fastest is
+ * kept for the scene's later gpr_trace calls.  Results do not depend on the
+ * configuration.  No counterpart in the reference (its traversal launch shape
+ * is fixed, pathtracer_kernel.h:235-246). */
+int gpr_trace_tune(gpr_scene* s, const gpr_ray* rays, uint64_t n, uint32_t flags, gpr_trace_cfg* out,
+                   uint32_t cap, uint32_t* n_out);
+int gpr_trace_set_cfg(gpr_scene* s, const gpr_trace_cfg* cfg);
 /* Node visits and triangle tests of the last GPR_TRACE_REF_ORDER call. */
 int gpr_trace_ref_stats(const gpr_scene* s, uint64_t* node_visits, uint64_t* tri_tests);
 
@@ -234,14 +252,6 @@ typedef struct {
     float c2w[16];      /* row-major cameraToWorld */
 } gpr_job;
 
-/* Traversal launch configuration picked by the autotuner. */
-typedef struct {
-    uint32_t block;     /* threads per block of the extend / shadow kernels */
-    uint32_t stack;     /* LDS stack entries per lane */
-    uint32_t chunk;     /* queue entries claimed per wave per atomic */
-    uint32_t layout;    /* 0 BVH4 128-B nodes, 1 reference DFS layout */
-    float ms;           /* time of the probe pass with this configuration */
-} gpr_trace_cfg;
 
 typedef struct gpr_batch gpr_batch;
 typedef struct {

@@ -102,3 +102,27 @@ def test_axis_parallel_and_degenerate_rays():
     ref, _, _ = O.trace(onodes, pos, rays)
     assert np.array_equal(s.trace(rays, ref_order=True).view(np.uint32), ref.view(np.uint32))
     compare_fast(s.trace(rays), ref, pos, rays, "axis")
+
+
+def test_launch_configurations_give_identical_hits():
+    import torch
+    pos = gpr.gen_soup(5, 0, 20000, 0.01)
+    rays = gpr.gen_rays(5, 1, 50000)
+    s = gpr.Scene.soup(pos).commit()
+    d_rays = torch.from_numpy(rays.view(np.uint8)).cuda()
+    d_hits = torch.empty(len(rays) * 16, dtype=torch.uint8, device="cuda")
+    ref = None
+    print("[trace] launch configurations", flush=True)
+    for cfg in [(256, 16, 64), (64, 8, 64), (512, 32, 256), (128, 16, 128)]:
+        s.trace_set_cfg(*cfg)
+        s.trace_device(d_rays.data_ptr(), d_hits.data_ptr(), len(rays))
+        torch.cuda.synchronize()
+        h = d_hits.cpu().numpy().copy()
+        if ref is None:
+            ref = h
+        assert np.array_equal(h, ref), cfg
+    res = s.trace_tune(d_rays.data_ptr(), len(rays))
+    assert len(res) == 36 and all(res[k].ms <= res[k + 1].ms for k in range(35))
+    s.trace_device(d_rays.data_ptr(), d_hits.data_ptr(), len(rays))
+    torch.cuda.synchronize()
+    assert np.array_equal(d_hits.cpu().numpy(), ref)
