@@ -408,8 +408,8 @@ def main():
 
     # ---- traversal launch configuration: autotuned on the batch itself
     print("[bench] traversal autotune", file=sys.stderr, flush=True)
-    tuned_any = scene.trace_tune(d_shadow.data_ptr(), N_RAYS, any_hit=True)
-    tuned = scene.trace_tune(d_rays.data_ptr(), N_RAYS)
+    tuned_any = scene.trace_tune(d_shadow.data_ptr(), N_RAYS, any_hit=True, cap=128)
+    tuned = scene.trace_tune(d_rays.data_ptr(), N_RAYS, cap=128)
 
     # ---- closest hit: warmup, then exactly K timed steps
     print("[bench] closest-hit timing", file=sys.stderr, flush=True)
@@ -427,7 +427,7 @@ def main():
     hit_frac = float((hits["id"] != -1).mean())
 
     # ---- any-hit shadow pass (its own tuned configuration)
-    scene.trace_set_cfg(tuned_any[0].block, tuned_any[0].stack, tuned_any[0].chunk)
+    scene.trace_set_cfg(tuned_any[0].block, tuned_any[0].stack, tuned_any[0].chunk, tuned_any[0].refill)
     for _ in range(a.warmup):
         scene.trace_device(d_shadow.data_ptr(), d_hits.data_ptr(), N_RAYS, any_hit=True)
     torch.cuda.synchronize()
@@ -457,8 +457,11 @@ def main():
         "config": {"workload": "lbvh_build_ray_query_1M_tris_4M_rays", "triangles": N_TRI,
                    "rays": N_RAYS, "tmax": "inf", "traversal": "bvh4_128B_lds_stack",
                    "tuned": {"block": tuned[0].block, "stack": tuned[0].stack, "chunk": tuned[0].chunk,
-                             "any_hit": [tuned_any[0].block, tuned_any[0].stack, tuned_any[0].chunk],
-                             "default_256_16_64_ms": round(next(c.ms for c in tuned if (c.block, c.stack, c.chunk) == (256, 16, 64)), 4)}},
+                             "refill": tuned[0].refill,
+                             "any_hit": [tuned_any[0].block, tuned_any[0].stack, tuned_any[0].chunk,
+                                         tuned_any[0].refill],
+                             "default_256_16_64_ms": round(next(c.ms for c in tuned if (c.block, c.stack, c.chunk, c.refill)
+                                                                == (256, 16, 64, 0)), 4)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": pmc_traffic("k_trace_fast<false>"),

@@ -484,13 +484,14 @@ int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint
         return GPR_OK;
     }
     if (flags & GPR_TRACE_DEVICE_PTRS)
-        return launch_trace(sv, rays, hits, n, flags, stats, stream(), s->trace_nt, s->trace_sl, s->trace_chunk);
+        return launch_trace(sv, rays, hits, n, flags, stats, stream(), s->trace_nt, s->trace_sl, s->trace_chunk,
+                            s->trace_refill);
     DevBuf dr, dh;
     GPR_TRY(dr.alloc(n * sizeof(gpr_ray)));
     GPR_TRY(dh.alloc(n * sizeof(gpr_hit)));
     GPR_HIP(hipMemcpyAsync(dr.p, rays, n * sizeof(gpr_ray), hipMemcpyHostToDevice, stream()));
     GPR_TRY(launch_trace(sv, dr.as<gpr_ray>(), dh.as<gpr_hit>(), n, flags, stats, stream(), s->trace_nt,
-                         s->trace_sl, s->trace_chunk));
+                         s->trace_sl, s->trace_chunk, s->trace_refill));
     GPR_HIP(hipMemcpyAsync(hits, dh.p, n * sizeof(gpr_hit), hipMemcpyDeviceToHost, stream()));
     GPR_HIP(hipStreamSynchronize(stream()));
     unsigned long long ov = 0;
@@ -508,6 +509,7 @@ int gpr_trace_set_cfg(gpr_scene* s, const gpr_trace_cfg* c) {
     s->trace_nt = (int)c->block;
     s->trace_sl = (int)c->stack;
     s->trace_chunk = (int)c->chunk;
+    s->trace_refill = c->refill ? 1 : 0;
     return GPR_OK;
 }
 
@@ -526,20 +528,22 @@ int gpr_trace_tune(gpr_scene* s, const gpr_ray* rays, uint64_t n, uint32_t flags
     GPR_HIP(hipEventCreate(&e1));
     std::vector<gpr_trace_cfg> res;
     int rc = GPR_OK;
-    for (int nt : {64, 128, 256, 512})
-        for (int sl : {8, 16, 32})
-            for (int chunk : {64, 128, 256}) {
-                if (rc != GPR_OK) break;
-                rc = launch_trace(sv, rays, hits.as<gpr_hit>(), n, flags, nullptr, st, nt, sl, chunk);
-                if (rc == GPR_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(GPR_ERR_HIP, "tune: event");
-                for (int r = 0; r < 2 && rc == GPR_OK; ++r)
-                    rc = launch_trace(sv, rays, hits.as<gpr_hit>(), n, flags, nullptr, st, nt, sl, chunk);
-                float ms = 0;
-                if (rc == GPR_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                    rc = fail(GPR_ERR_HIP, "tune: event timing");
-                if (rc == GPR_OK) res.push_back(gpr_trace_cfg{(uint32_t)nt, (uint32_t)sl, (uint32_t)chunk, 0, ms / 2});
-            }
+    for (int refill : {0, 1})
+        for (int nt : {64, 128, 256, 512})
+            for (int sl : {8, 16, 32})
+                for (int chunk : {64, 128, 256}) {
+                    if (rc != GPR_OK) break;
+                    rc = launch_trace(sv, rays, hits.as<gpr_hit>(), n, flags, nullptr, st, nt, sl, chunk, refill);
+                    if (rc == GPR_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(GPR_ERR_HIP, "tune: event");
+                    for (int r = 0; r < 2 && rc == GPR_OK; ++r)
+                        rc = launch_trace(sv, rays, hits.as<gpr_hit>(), n, flags, nullptr, st, nt, sl, chunk, refill);
+                    float ms = 0;
+                    if (rc == GPR_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                        rc = fail(GPR_ERR_HIP, "tune: event timing");
+                    if (rc == GPR_OK)
+                        res.push_back(gpr_trace_cfg{(uint32_t)nt, (uint32_t)sl, (uint32_t)chunk, 0, (uint32_t)refill, ms / 2});
+                }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     GPR_TRY(rc);
@@ -550,6 +554,7 @@ int gpr_trace_tune(gpr_scene* s, const gpr_ray* rays, uint64_t n, uint32_t flags
     s->trace_nt = (int)res[0].block;
     s->trace_sl = (int)res[0].stack;
     s->trace_chunk = (int)res[0].chunk;
+    s->trace_refill = (int)res[0].refill;
     if (nout) *nout = (uint32_t)std::min<size_t>(cap, res.size());
     if (out)
         for (uint32_t k = 0; k < cap && k < res.size(); ++k) out[k] = res[k];

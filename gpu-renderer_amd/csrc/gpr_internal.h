@@ -147,7 +147,7 @@ struct gpr_scene {
     bool committed = false;
     float r2c[16];
     uint64_t last_nv = 0, last_nt = 0;
-    int trace_nt = 256, trace_sl = 16, trace_chunk = 64;  // BVH4 traversal launch configuration
+    int trace_nt = 256, trace_sl = 16, trace_chunk = 64, trace_refill = 0;  // BVH4 traversal launch configuration
     gpr::SceneView view() const;
 };
 
@@ -162,7 +162,8 @@ int sort_prepare(void* scratch, size_t bytes, hipStream_t st);
 // trace.hip
 // nt / sl / chunk: block size, LDS stack entries, rays claimed per wave (the autotuner's space)
 int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags,
-                 unsigned long long* stats, hipStream_t st, int nt = 256, int sl = 16, int chunk = 64);
+                 unsigned long long* stats, hipStream_t st, int nt = 256, int sl = 16, int chunk = 64,
+                 int refill = 0);
 bool trace_cfg_valid(int nt, int sl, int chunk);
 // host math shared by api.cpp and the renderer
 void camera_raster_to_camera(uint32_t w, uint32_t h, float fov_deg, float out[16]);

@@ -127,7 +127,7 @@ int tune_engine(Engine* e, const std::vector<EngineJob>& pass, float4* fb, hipSt
                              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             rc = fail(GPR_ERR_HIP, "tune: event timing");
         if (rc != GPR_OK) break;
-        res.push_back(gpr_trace_cfg{(uint32_t)c.nt, (uint32_t)c.sl, (uint32_t)c.chunk, (uint32_t)c.layout, ms / 2});
+        res.push_back(gpr_trace_cfg{(uint32_t)c.nt, (uint32_t)c.sl, (uint32_t)c.chunk, (uint32_t)c.layout, 0, ms / 2});
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -58,6 +58,76 @@ __global__ __launch_bounds__(NT) void k_trace_fast(SceneView sv, const gpr_ray* 
     }
 }
 
+// Per-lane ray refill: the lanes of a wave step their rays one node at a
+// time and a lane whose ray is done takes the next one from a wave-local pool
+// of `chunk` ray indices (one global atomic per chunk), so no lane idles while
+// the slowest ray of the wave finishes.  Every ray takes the same node steps
+// as in k_trace_fast, so the hits are identical.
+template <bool ANY, int NT, int SL>
+__global__ __launch_bounds__(NT) void k_trace_refill(SceneView sv, const gpr_ray* __restrict__ rays,
+                                                     gpr_hit* __restrict__ hits, uint64_t n,
+                                                     unsigned long long* __restrict__ cursor,
+                                                     int2* __restrict__ spill, uint32_t chunk) {
+    __shared__ int2 lds[SL * NT];
+    LaneStackT<NT, SL> st;
+    st.lds = lds + threadIdx.x;
+    st.spill = spill + (uint64_t)(blockIdx.x * NT + threadIdx.x) * kStackGlobal;
+    st.sp = 0;
+    const int lane = threadIdx.x & 63;
+    uint64_t pb = 0, pe = 0;  // wave-local pool [pb, pe) of ray indices (wave-uniform)
+    bool drained = false;     // the global cursor has passed n (wave-uniform)
+    bool active = false;
+    uint64_t ri = 0;
+    Trav t;
+    while (true) {
+        const unsigned long long need = __ballot(!active);
+        if (need != 0ull && !(drained && pb == pe)) {
+            const uint32_t cnt = (uint32_t)__popcll(need);
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+            const uint64_t avail = pe - pb;
+            uint64_t nb = pe, ne = pe;
+            if (avail < cnt && !drained) {
+                unsigned long long b = 0;
+                if (lane == 0) b = atomicAdd(cursor, (unsigned long long)chunk);
+                b = __shfl(b, 0, 64);
+                nb = b < n ? b : n;
+                ne = b + chunk < n ? b + chunk : n;
+                if (b + chunk >= n) drained = true;
+            }
+            if (!active) {
+                const bool from_old = rank < avail;
+                const uint64_t idx = from_old ? pb + rank : nb + (rank - avail);
+                if (from_old || idx < ne) {
+                    ri = idx;
+                    const Ray3 r = load_ray(rays, ri);
+                    if (trav_begin<ANY>(sv, r, st, t)) store_hit(hits, ri, t.hit);
+                    else active = true;
+                }
+            }
+            if (avail >= cnt) {
+                pb += cnt;
+            } else {
+                const uint64_t taken = nb + (cnt - avail);
+                pb = taken < ne ? taken : ne;
+                pe = ne;
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (drained && pb == pe) break;
+            continue;
+        }
+        if (active) {
+            const int rc = trav_step<ANY>(sv, st, t);
+            if (rc != kTravMore) {
+                if (rc == kTravOverflow) atomicAdd(cursor + 1, 1ull);  // stack overflow count
+                store_hit(hits, ri, t.hit);
+                active = false;
+            }
+        }
+    }
+}
+
 template <bool ANY>
 __global__ __launch_bounds__(256) void k_trace_ref(SceneView sv, const gpr_ray* __restrict__ rays,
                                                    gpr_hit* __restrict__ hits, uint64_t n,
@@ -83,12 +153,14 @@ __global__ __launch_bounds__(256) void k_trace_ref(SceneView sv, const gpr_ray* 
 
 using TraceKernel = void (*)(SceneView, const gpr_ray*, gpr_hit*, uint64_t, unsigned long long*, int2*, uint32_t);
 struct TraceVariant {
-    TraceKernel closest, any;
+    TraceKernel closest, any;          // wave-synchronous rays
+    TraceKernel closest_r, any_r;      // per-lane ray refill
     int nt, sl;
 };
 template <int NT, int SL>
 TraceVariant trace_variant() {
-    return {k_trace_fast<false, NT, SL>, k_trace_fast<true, NT, SL>, NT, SL};
+    return {k_trace_fast<false, NT, SL>, k_trace_fast<true, NT, SL>, k_trace_refill<false, NT, SL>,
+            k_trace_refill<true, NT, SL>, NT, SL};
 }
 constexpr int kTraceVariants = 12;
 const TraceVariant& trace_variant_at(int k) {
@@ -109,7 +181,7 @@ int trace_variant_index(int nt, int sl) {
 struct TraceResources {
     DevBuf spill;
     DevBuf cursor;
-    int grid[kTraceVariants][2] = {};
+    int grid[kTraceVariants][4] = {};  // [variant][closest, any, closest refill, any refill]
     int dev = -1;
 };
 TraceResources g_res[16];
@@ -122,9 +194,10 @@ int ensure_resources(TraceResources& R) {
     size_t threads = 0;
     for (int k = 0; k < kTraceVariants; ++k) {
         const TraceVariant& v = trace_variant_at(k);
-        for (int a = 0; a < 2; ++a) {
+        const TraceKernel ks[4] = {v.closest, v.any, v.closest_r, v.any_r};
+        for (int a = 0; a < 4; ++a) {
             int per_cu = 0;
-            GPR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, a ? v.any : v.closest, v.nt, 0));
+            GPR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ks[a], v.nt, 0));
             R.grid[k][a] = cus * std::max(1, per_cu);
             threads = std::max(threads, (size_t)R.grid[k][a] * v.nt);
         }
@@ -142,7 +215,7 @@ bool trace_cfg_valid(int nt, int sl, int chunk) {
 }
 
 int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags,
-                 unsigned long long* stats, hipStream_t st, int nt, int sl, int chunk) {
+                 unsigned long long* stats, hipStream_t st, int nt, int sl, int chunk, int refill) {
     if (n == 0) return GPR_OK;
     const bool any = (flags & GPR_TRACE_ANY) != 0;
     if (flags & GPR_TRACE_REF_ORDER) {
@@ -162,9 +235,11 @@ int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64
     unsigned long long* cur = R.cursor.as<unsigned long long>();
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cur, 4);
     const uint64_t want = (n + v.nt - 1) / v.nt;  // more blocks than rays would only idle
-    const dim3 grid((unsigned)std::min<uint64_t>(R.grid[k][any ? 1 : 0], want));
-    hipLaunchKernelGGL(any ? v.any : v.closest, grid, dim3(v.nt), 0, st, sv, rays, hits, n, cur,
-                       R.spill.as<int2>(), (uint32_t)chunk);
+    const int kind = (refill ? 2 : 0) + (any ? 1 : 0);
+    const TraceKernel ks[4] = {v.closest, v.any, v.closest_r, v.any_r};
+    const dim3 grid((unsigned)std::min<uint64_t>(R.grid[k][kind], want));
+    hipLaunchKernelGGL(ks[kind], grid, dim3(v.nt), 0, st, sv, rays, hits, n, cur, R.spill.as<int2>(),
+                       (uint32_t)chunk);
     GPR_HIP(hipGetLastError());
     return GPR_OK;
 }

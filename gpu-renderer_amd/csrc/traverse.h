@@ -156,91 +156,120 @@ GPR_D void tri_rec_hit(const TriRec* tris, uint32_t ntri, int32_t leaf, const Ra
     }
 }
 
-// Closest hit (ANY=false) or any hit (ANY=true) against the BVH4.  Returns
-// false only if the stack capacity was exceeded (the ray is then reported
-// as a miss and the caller raises an error flag).
+// BVH4 traversal as an explicit state machine: trav_begin sets a ray up,
+// each trav_step visits one node (its leaf children are tested first, they
+// tighten tmax before the descent decision), so a caller can either run a ray
+// to completion (fast_trace) or interleave the steps of many rays and give a
+// lane a new ray as soon as its ray is done.
+struct Trav {
+    Ray3 r;
+    v3 inv;
+    float tmax;
+    gpr_hit hit;
+    int32_t node;
+    uint32_t fetched;
+    bool found;
+};
+constexpr float kSlabCons = 1.0f + 2.0f * 1.7881394e-07f;  // 1 + 2*gamma(3)
+enum : int { kTravMore = 0, kTravDone = 1, kTravOverflow = 2 };
+
+// Returns true when the ray is already finished (single-triangle scene).
 template <bool ANY, class Stack>
-GPR_D bool fast_trace(const SceneView& sv, const Ray3& r, Stack& st, gpr_hit& hit) {
-    hit = gpr_hit{INFINITY, -1, 0.f, 0.f};
-    float tmax = r.tmax;
-    bool found = false;
-    if (sv.root4 < 0) {  // single triangle
-        tri_rec_hit(sv.tris, sv.ntri, ~sv.root4, r, tmax, hit, found);
+GPR_D bool trav_begin(const SceneView& sv, const Ray3& r, Stack& st, Trav& t) {
+    t.r = r;
+    t.hit = gpr_hit{INFINITY, -1, 0.f, 0.f};
+    t.tmax = r.tmax;
+    t.found = false;
+    t.fetched = 0;
+    st.sp = 0;
+    if (sv.root4 < 0) {
+        tri_rec_hit(sv.tris, sv.ntri, ~sv.root4, r, t.tmax, t.hit, t.found);
         return true;
     }
-    const v3 inv = V(1 / r.d.x, 1 / r.d.y, 1 / r.d.z);
-    const float kCons = 1.0f + 2.0f * 1.7881394e-07f;  // 1 + 2*gamma(3)
-    int32_t node = sv.root4;
-    st.sp = 0;
+    t.inv = V(1 / r.d.x, 1 / r.d.y, 1 / r.d.z);
+    t.node = sv.root4;
+    return false;
+}
+
+template <bool ANY, class Stack>
+GPR_D int trav_step(const SceneView& sv, Stack& st, Trav& t) {
     // a depth-first walk fetches every node at most once: the cap only trips on a malformed tree
-    for (uint32_t fetched = 0;; ++fetched) {
-        if ((uint32_t)node >= sv.nbvh4 || fetched > sv.nbvh4) return false;
-        const float4* np = reinterpret_cast<const float4*>(sv.bvh4 + node);
-        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-        const int4 ch = *reinterpret_cast<const int4*>(np + 6);
-        const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-        const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-        const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-        const int32_t cha[4] = {ch.x, ch.y, ch.z, ch.w};
-        float tn[4];
-        bool hitc[4];
+    if ((uint32_t)t.node >= sv.nbvh4 || t.fetched > sv.nbvh4) return kTravOverflow;
+    const Ray3& r = t.r;
+    const float4* np = reinterpret_cast<const float4*>(sv.bvh4 + t.node);
+    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+    const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+    const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+    const int32_t cha[4] = {ch.x, ch.y, ch.z, ch.w};
+    float tn[4];
+    bool hitc[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const float x0 = (lxa[c] - r.o.x) * inv.x, x1 = (hxa[c] - r.o.x) * inv.x;
-            const float y0 = (lya[c] - r.o.y) * inv.y, y1 = (hya[c] - r.o.y) * inv.y;
-            const float z0 = (lza[c] - r.o.z) * inv.z, z1 = (hza[c] - r.o.z) * inv.z;
-            const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
-            const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
-            tn[c] = tnear;
-            hitc[c] = tnear <= tfar * kCons && cha[c] != kEmptyChild;
-        }
-        // leaf children first: they tighten tmax before the descent decision
+    for (int c = 0; c < 4; ++c) {
+        const float x0 = (lxa[c] - r.o.x) * t.inv.x, x1 = (hxa[c] - r.o.x) * t.inv.x;
+        const float y0 = (lya[c] - r.o.y) * t.inv.y, y1 = (hya[c] - r.o.y) * t.inv.y;
+        const float z0 = (lza[c] - r.o.z) * t.inv.z, z1 = (hza[c] - r.o.z) * t.inv.z;
+        const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+        const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t.tmax));
+        tn[c] = tnear;
+        hitc[c] = tnear <= tfar * kSlabCons && cha[c] != kEmptyChild;
+    }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (hitc[c] && cha[c] < 0) {
-                tri_rec_hit(sv.tris, sv.ntri, ~cha[c], r, tmax, hit, found);
-                if (ANY && found) return true;
-            }
+    for (int c = 0; c < 4; ++c) {
+        if (hitc[c] && cha[c] < 0) {
+            tri_rec_hit(sv.tris, sv.ntri, ~cha[c], r, t.tmax, t.hit, t.found);
+            if (ANY && t.found) return kTravDone;
         }
-        uint32_t key[4];
+    }
+    uint32_t key[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const bool go = hitc[c] && cha[c] >= 0 && tn[c] <= tmax * kCons;
-            key[c] = go ? ((__float_as_uint(fmaxf(tn[c], 0.f)) & ~3u) | (uint32_t)c) : 0xffffffffu;
-        }
-        // 4-key sorting network, ascending
-#define GPR_CSWAP(a, b)                   \
-    {                                     \
+    for (int c = 0; c < 4; ++c) {
+        const bool go = hitc[c] && cha[c] >= 0 && tn[c] <= t.tmax * kSlabCons;
+        key[c] = go ? ((__float_as_uint(fmaxf(tn[c], 0.f)) & ~3u) | (uint32_t)c) : 0xffffffffu;
+    }
+    // 4-key sorting network, ascending
+#define GPR_CSWAP(a, b)                           \
+    {                                             \
         const uint32_t lo_ = min(key[a], key[b]); \
         const uint32_t hi_ = max(key[a], key[b]); \
-        key[a] = lo_;                     \
-        key[b] = hi_;                     \
+        key[a] = lo_;                             \
+        key[b] = hi_;                             \
     }
-        GPR_CSWAP(0, 1) GPR_CSWAP(2, 3) GPR_CSWAP(0, 2) GPR_CSWAP(1, 3) GPR_CSWAP(1, 2)
+    GPR_CSWAP(0, 1) GPR_CSWAP(2, 3) GPR_CSWAP(0, 2) GPR_CSWAP(1, 3) GPR_CSWAP(1, 2)
 #undef GPR_CSWAP
-        int cnt = (key[0] != 0xffffffffu) + (key[1] != 0xffffffffu) + (key[2] != 0xffffffffu) +
-                  (key[3] != 0xffffffffu);
-        if (cnt > 0) {
-            for (int k = cnt - 1; k >= 1; --k) {
-                const int c = key[k] & 3;
-                if (!st.push(make_int2(cha[c], __float_as_int(tn[c])))) return false;
-            }
-            node = cha[key[0] & 3];
-            continue;
+    const int cnt = (key[0] != 0xffffffffu) + (key[1] != 0xffffffffu) + (key[2] != 0xffffffffu) +
+                    (key[3] != 0xffffffffu);
+    ++t.fetched;
+    if (cnt > 0) {
+        for (int k = cnt - 1; k >= 1; --k) {
+            const int c = key[k] & 3;
+            if (!st.push(make_int2(cha[c], __float_as_int(tn[c])))) return kTravOverflow;
         }
-        // pop the nearest pending node that can still hold a closer hit
-        bool next = false;
-        while (st.sp > 0) {
-            const int2 e = st.pop();
-            if (ANY || __int_as_float(e.y) <= tmax * kCons) {
-                node = e.x;
-                next = true;
-                break;
-            }
-        }
-        if (!next) break;
+        t.node = cha[key[0] & 3];
+        return kTravMore;
     }
-    return true;
+    // pop the nearest pending node that can still hold a closer hit
+    while (st.sp > 0) {
+        const int2 e = st.pop();
+        if (ANY || __int_as_float(e.y) <= t.tmax * kSlabCons) {
+            t.node = e.x;
+            return kTravMore;
+        }
+    }
+    return kTravDone;
+}
+
+// Closest hit (ANY=false) or any hit (ANY=true) against the BVH4.  Returns
+// false only if the stack capacity was exceeded (the ray is then reported
+// with the hits found so far and the caller raises an error flag).
+template <bool ANY, class Stack>
+GPR_D bool fast_trace(const SceneView& sv, const Ray3& r, Stack& st, gpr_hit& hit) {
+    Trav t;
+    int rc = trav_begin<ANY>(sv, r, st, t) ? kTravDone : kTravMore;
+    while (rc == kTravMore) rc = trav_step<ANY>(sv, st, t);
+    hit = t.hit;
+    return rc != kTravOverflow;
 }
 
 }  // namespace gpr

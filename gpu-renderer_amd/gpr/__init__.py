@@ -60,11 +60,11 @@ class Job(C.Structure):
 
 class TraceCfg(C.Structure):
     _fields_ = [("block", C.c_uint32), ("stack", C.c_uint32), ("chunk", C.c_uint32),
-                ("layout", C.c_uint32), ("ms", C.c_float)]
+                ("layout", C.c_uint32), ("refill", C.c_uint32), ("ms", C.c_float)]
 
     def __repr__(self):
         return (f"TraceCfg(block={self.block}, stack={self.stack}, chunk={self.chunk}, "
-                f"layout={self.layout}, ms={self.ms:.3f})")
+                f"layout={self.layout}, refill={self.refill}, ms={self.ms:.3f})")
 
 
 class BatchOpts(C.Structure):
@@ -371,8 +371,8 @@ class Scene:
         check(lib.gpr_trace_tune(self.h, C.c_void_p(rays_ptr), n, fl, out, cap, C.byref(k)))
         return [out[i] for i in range(k.value)]
 
-    def trace_set_cfg(self, block, stack, chunk):
-        c = TraceCfg(block, stack, chunk, 0, 0.0)
+    def trace_set_cfg(self, block, stack, chunk, refill=0):
+        c = TraceCfg(block, stack, chunk, 0, refill, 0.0)
         check(lib.gpr_trace_set_cfg(self.h, C.byref(c)))
 
     def ref_stats(self):
@@ -492,7 +492,7 @@ class Batch:
         return [out[k] for k in range(n.value)]
 
     def set_cfg(self, block, stack, chunk, layout=0):
-        c = TraceCfg(block, stack, chunk, layout, 0.0)
+        c = TraceCfg(block, stack, chunk, layout, 0, 0.0)
         check(lib.gpr_batch_set_cfg(self.h, C.byref(c)))
 
     def run(self):
@@ -544,7 +544,7 @@ class Server:
         return [out[k] for k in range(n.value)]
 
     def set_cfg(self, block, stack, chunk, layout=0):
-        c = TraceCfg(block, stack, chunk, layout, 0.0)
+        c = TraceCfg(block, stack, chunk, layout, 0, 0.0)
         check(lib.gpr_server_set_cfg(self.h, C.byref(c)))
 
     def submit(self, j):

@@ -185,6 +185,7 @@ typedef struct {
     uint32_t stack;     /* LDS stack entries per lane: 8, 16, 32 */
     uint32_t chunk;     /* rays / queue entries claimed per wave per atomic */
     uint32_t layout;    /* 0 BVH4 128-B nodes, 1 reference DFS layout (batched / served only) */
+    uint32_t refill;    /* gpr_trace: 1 = a lane takes a new ray as soon as its ray is done */
     float ms;           /* time of the probe with this configuration */
 } gpr_trace_cfg;
 #define GPR_TRACE_ANY 1u         /* any-hit: stop at the first accepted hit */
@@ -194,7 +195,8 @@ typedef struct {
 int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags);
 /* Autotune the BVH4 traversal of gpr_trace for this scene on a batch of device
  * rays (flags: GPR_TRACE_DEVICE_PTRS, optionally GPR_TRACE_ANY): block size
- * {64,128,256,512} x LDS stack {8,16,32} x chunk {64,128,256}; the fastest is
+ * {64,128,256,512} x LDS stack {8,16,32} x chunk {64,128,256} x per-lane ray
+ * refill {off, on}; the fastest is
  * kept for the scene's later gpr_trace calls.  Results do not depend on the
  * configuration.  No counterpart in the reference (its traversal launch shape
  * is fixed, pathtracer_kernel.h:235-246). */

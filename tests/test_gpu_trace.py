@@ -113,7 +113,8 @@ def test_launch_configurations_give_identical_hits():
     d_hits = torch.empty(len(rays) * 16, dtype=torch.uint8, device="cuda")
     ref = None
     print("[trace] launch configurations", flush=True)
-    for cfg in [(256, 16, 64), (64, 8, 64), (512, 32, 256), (128, 16, 128)]:
+    for cfg in [(256, 16, 64, 0), (64, 8, 64, 0), (512, 32, 256, 0), (128, 16, 128, 0),
+                (256, 16, 64, 1), (64, 8, 128, 1), (512, 32, 64, 1)]:
         s.trace_set_cfg(*cfg)
         s.trace_device(d_rays.data_ptr(), d_hits.data_ptr(), len(rays))
         torch.cuda.synchronize()
@@ -121,8 +122,19 @@ def test_launch_configurations_give_identical_hits():
         if ref is None:
             ref = h
         assert np.array_equal(h, ref), cfg
-    res = s.trace_tune(d_rays.data_ptr(), len(rays))
-    assert len(res) == 36 and all(res[k].ms <= res[k + 1].ms for k in range(35))
+    # any hit: occlusion identical across configurations
+    occ = None
+    for cfg in [(256, 16, 64, 0), (128, 8, 64, 1)]:
+        s.trace_set_cfg(*cfg)
+        s.trace_device(d_rays.data_ptr(), d_hits.data_ptr(), len(rays), any_hit=True)
+        torch.cuda.synchronize()
+        o = d_hits.cpu().numpy().view(gpr.HIT_DTYPE)["id"] != -1
+        if occ is None:
+            occ = o
+        assert np.array_equal(o, occ), cfg
+    s.trace_set_cfg(256, 16, 64, 0)
+    res = s.trace_tune(d_rays.data_ptr(), len(rays), cap=128)
+    assert len(res) == 72 and all(res[k].ms <= res[k + 1].ms for k in range(71))
     s.trace_device(d_rays.data_ptr(), d_hits.data_ptr(), len(rays))
     torch.cuda.synchronize()
     assert np.array_equal(d_hits.cpu().numpy(), ref)
