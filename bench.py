@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--scenes", type=int, default=0, help="scenes of the batch / serve workloads (default 256 / 64)")
     ap.add_argument("--no-serve", action="store_true", help="raycast run without the served-stream segment")
     ap.add_argument("--capacity", type=int, default=1 << 23, help="paths per pass of the served / batched engine")
+    ap.add_argument("--trace-cfg", default="128,16,64,1",
+                    help="block,stack,chunk,refill of the --only trace profiling run (the tuned configuration)")
     return ap.parse_args()
 
 
@@ -348,6 +350,7 @@ def main():
     torch.cuda.synchronize()
 
     if a.only == "trace":
+        scene.trace_set_cfg(*[int(x) for x in a.trace_cfg.split(",")])
         for _ in range(a.warmup + a.steps):
             scene.trace_device(d_rays.data_ptr(), d_hits.data_ptr(), N_RAYS)
         torch.cuda.synchronize()
@@ -464,8 +467,8 @@ def main():
                                                                 == (256, 16, 64, 0)), 4)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic("k_trace_fast<false>"),
-                     "kernel": "k_trace_fast<false>",
+                     "traffic": pmc_traffic("closest_hit_tuned"),
+                     "kernel": f"k_trace_{'refill' if tuned[0].refill else 'fast'}<false, {tuned[0].block}, {tuned[0].stack}>",
                      "bytes_per_ray": round(bytes_per_ray, 2),
                      "mean_nodes": round(n_node, 3), "mean_tris": round(n_tri, 3)},
         "roofline_sort": {"bound": "hbm", "achieved": round(sort_bytes / (sort_ms * 1e-3) / 1e9, 1),
