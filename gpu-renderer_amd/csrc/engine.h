@@ -37,6 +37,7 @@ struct TraceCfg {
     int sl;      // LDS stack entries per lane: 8, 16, 32
     int chunk;   // queue entries a wave claims per atomic: 64 .. 512
     int layout;  // 0 = BVH4 128-B nodes, 1 = reference DFS layout (BVH2)
+    int refill;  // 1 = a lane takes a new queue entry as soon as its ray is done (BVH4 only)
 };
 std::vector<TraceCfg> engine_tune_space();
 

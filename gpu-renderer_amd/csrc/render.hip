@@ -264,8 +264,110 @@ GPR_D void for_each_claimed(uint32_t* cursor, uint32_t n, uint32_t chunk, F&& f)
     }
 }
 
-// extend: closest hit; for scattered rays also the BSDF-sampling MIS term
-template <bool PRIMARY, bool REF_ORDER, bool MULTI, int NT, int SL>
+// Per-lane ray refill over queue entries: a lane whose ray is done takes the
+// next entry from a wave-local pool of `chunk` entries (one atomic per chunk).
+// begin(i) sets entry i up and returns true if it finished at once; step()
+// advances the lane's ray one node and returns true when it is done.
+template <class Begin, class Step>
+GPR_D void refill_loop(uint32_t* cursor, uint32_t n, uint32_t chunk, Begin&& begin, Step&& step) {
+    const int lane = threadIdx.x & 63;
+    uint32_t pb = 0, pe = 0;  // wave-local pool of entries (wave-uniform)
+    bool drained = false;
+    bool active = false;
+    while (true) {
+        const unsigned long long need = __ballot(!active);
+        if (need != 0ull && !(drained && pb == pe)) {
+            const uint32_t cnt = (uint32_t)__popcll(need);
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+            const uint32_t avail = pe - pb;
+            uint32_t nb = pe, ne = pe;
+            if (avail < cnt && !drained) {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(cursor, chunk);
+                b = __shfl(b, 0, 64);
+                nb = b < n ? b : n;
+                ne = (uint64_t)b + chunk < n ? b + chunk : n;
+                if ((uint64_t)b + chunk >= n) drained = true;
+            }
+            if (!active) {
+                const bool from_old = rank < avail;
+                const uint32_t idx = from_old ? pb + rank : nb + (rank - avail);
+                if (from_old || idx < ne) active = !begin(idx);
+            }
+            if (avail >= cnt) {
+                pb += cnt;
+            } else {
+                const uint32_t taken = nb + (cnt - avail);
+                pb = taken < ne ? taken : ne;
+                pe = ne;
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (drained && pb == pe) break;
+            continue;
+        }
+        if (active) active = !step();
+    }
+}
+
+GPR_D Ray3 queue_ray(const QueueView& q, uint32_t i) {
+    const float4 ot = q.o_tmin[i], dt = q.d_tmax[i];
+    Ray3 r;
+    r.o = V(ot.x, ot.y, ot.z);
+    r.d = V(dt.x, dt.y, dt.z);
+    r.tmin = ot.w;
+    r.tmax = dt.w;
+    return r;
+}
+
+// After the closest hit of entry i: for scattered rays the BSDF-sampling MIS
+// term and, in reference RNG mode, its roulette (pathtracer_kernel.h:684-749).
+template <bool PRIMARY, bool MULTI>
+GPR_D void extend_finish(const RenderArgs& a, const QueueView& q, uint32_t i, const SceneView& sv, const Ray3& r,
+                         gpr_hit h, uint32_t bounce) {
+    if (!PRIMARY) {
+        const float4 wp = q.w_pdf[i];
+        const uint32_t pw = q.pix[i];
+        const uint32_t pix = pw & 0x7fffffffu;
+        const bool spec = (pw >> 31) != 0;
+        const c3 w = C(wp.x, wp.y, wp.z);
+        float u_rr = 0.f;
+        if (!MULTI && a.rng_mode == GPR_RNG_REFERENCE) u_rr = lcg_next(q.seed[i]);
+        if (h.id != -1 && !spec) {
+            const uint32_t t = (uint32_t)h.id;
+            const c3 le = emission_of(sv, t);
+            if (!black(le)) {
+                const Geom lg = setup_geom(sv, t, h.u, h.v);
+                float lpdf = 1.f / tri_area(sv, t);
+                lpdf /= sv.nlights;
+                const float d = h.t;
+                lpdf *= d * d / fabsf(dot3(r.d, lg.ns));
+                const c3 fl = dot3(-r.d, lg.ns) > 0 ? le : C(0.f);
+                const float wgt = power_heur(wp.w, lpdf);
+                fb_add(a.fb, pix, w * fl * wgt);
+            }
+        }
+        if (!MULTI && a.rng_mode == GPR_RNG_REFERENCE && cmax(w) < 1 && bounce > 3) {
+            // roulette without reweighting; a terminated path is a miss (pathtracer_kernel.h:738-748)
+            const float qq = fmax_h(0.05f, 1 - cmax(w));
+            if (u_rr < qq) h.id = -1;
+        }
+    }
+    q.hit[i] = make_float4(h.t, __int_as_float(h.id), h.u, h.v);
+}
+
+// After the any-hit test of shadow entry i (pathtracer_kernel.h:642-664).
+GPR_D void shadow_finish(const RenderArgs& a, const QueueView& q, uint32_t i, const gpr_hit& h) {
+    if (h.id == -1) {
+        const float4 wp = q.w_pdf[i];
+        fb_add(a.fb, q.pix[i] & 0x7fffffffu, C(wp.x, wp.y, wp.z));
+    }
+}
+
+// extend: closest hit; for scattered rays also the BSDF-sampling MIS term.
+// REFILL: per-lane ray refill (BVH4 traversal only).
+template <bool PRIMARY, bool REF_ORDER, bool MULTI, int NT, int SL, bool REFILL = false>
 __global__ __launch_bounds__(NT) void k_extend(RenderArgs a, uint32_t bounce) {
     __shared__ int2 lds[SL * NT];
     LaneStackT<NT, SL> st;
@@ -276,49 +378,38 @@ __global__ __launch_bounds__(NT) void k_extend(RenderArgs a, uint32_t bounce) {
     const QueueView& q = a.q[PRIMARY ? 0 : ((bounce + 1) & 1)];
     const uint32_t n = PRIMARY ? npaths_of<MULTI>(a) : a.ctrl->scatter_size[bounce];
     uint32_t* cur = &a.ctrl->ext_cursor[PRIMARY ? 0 : bounce + 1];
-    for_each_claimed(cur, n, a.chunk, [&](uint32_t i) {
-        const SceneView sv = scene_of<MULTI>(a, q, i);
-        const float4 ot = q.o_tmin[i], dt = q.d_tmax[i];
-        Ray3 r;
-        r.o = V(ot.x, ot.y, ot.z);
-        r.d = V(dt.x, dt.y, dt.z);
-        r.tmin = ot.w;
-        r.tmax = dt.w;
-        gpr_hit h = trace_one(a, sv, r, st, false, REF_ORDER);
-        if (!PRIMARY) {
-            const float4 wp = q.w_pdf[i];
-            const uint32_t pw = q.pix[i];
-            const uint32_t pix = pw & 0x7fffffffu;
-            const bool spec = (pw >> 31) != 0;
-            const c3 w = C(wp.x, wp.y, wp.z);
-            float u_rr = 0.f;
-            if (!MULTI && a.rng_mode == GPR_RNG_REFERENCE) u_rr = lcg_next(q.seed[i]);
-            if (h.id != -1 && !spec) {
-                const uint32_t t = (uint32_t)h.id;
-                const c3 le = emission_of(sv, t);
-                if (!black(le)) {
-                    const Geom lg = setup_geom(sv, t, h.u, h.v);
-                    float lpdf = 1.f / tri_area(sv, t);
-                    lpdf /= sv.nlights;
-                    const float d = h.t;
-                    lpdf *= d * d / fabsf(dot3(r.d, lg.ns));
-                    const c3 fl = dot3(-r.d, lg.ns) > 0 ? le : C(0.f);
-                    const float wgt = power_heur(wp.w, lpdf);
-                    fb_add(a.fb, pix, w * fl * wgt);
-                }
-            }
-            if (!MULTI && a.rng_mode == GPR_RNG_REFERENCE && cmax(w) < 1 && bounce > 3) {
-                // roulette without reweighting; a terminated path is a miss (pathtracer_kernel.h:738-748)
-                const float qq = fmax_h(0.05f, 1 - cmax(w));
-                if (u_rr < qq) h.id = -1;
-            }
-        }
-        q.hit[i] = make_float4(h.t, __int_as_float(h.id), h.u, h.v);
-    });
+    if constexpr (REFILL && !REF_ORDER) {
+        uint32_t ei = 0;
+        SceneView sv;
+        Trav t;
+        refill_loop(
+            cur, n, a.chunk,
+            [&](uint32_t i) {
+                ei = i;
+                sv = scene_of<MULTI>(a, q, i);
+                if (!trav_begin<false>(sv, queue_ray(q, i), st, t)) return false;
+                extend_finish<PRIMARY, MULTI>(a, q, i, sv, t.r, t.hit, bounce);
+                return true;
+            },
+            [&]() {
+                const int rc = trav_step<false>(sv, st, t);
+                if (rc == kTravMore) return false;
+                if (rc == kTravOverflow) atomicOr(&a.ctrl->error, 1u);
+                extend_finish<PRIMARY, MULTI>(a, q, ei, sv, t.r, t.hit, bounce);
+                return true;
+            });
+    } else {
+        for_each_claimed(cur, n, a.chunk, [&](uint32_t i) {
+            const SceneView sv = scene_of<MULTI>(a, q, i);
+            const Ray3 r = queue_ray(q, i);
+            const gpr_hit h = trace_one(a, sv, r, st, false, REF_ORDER);
+            extend_finish<PRIMARY, MULTI>(a, q, i, sv, r, h, bounce);
+        });
+    }
 }
 
 // shadow: any hit + accumulate_light_sampling_contribution (pathtracer_kernel.h:642-664)
-template <bool REF_ORDER, bool MULTI, int NT, int SL>
+template <bool REF_ORDER, bool MULTI, int NT, int SL, bool REFILL = false>
 __global__ __launch_bounds__(NT) void k_shadow(RenderArgs a, uint32_t bounce) {
     __shared__ int2 lds[SL * NT];
     LaneStackT<NT, SL> st;
@@ -327,20 +418,34 @@ __global__ __launch_bounds__(NT) void k_shadow(RenderArgs a, uint32_t bounce) {
     st.sp = 0;
     const QueueView& q = a.q[2];
     const uint32_t n = a.ctrl->shadow_size[bounce];
-    for_each_claimed(&a.ctrl->sh_cursor[bounce], n, a.chunk, [&](uint32_t i) {
-        const SceneView sv = scene_of<MULTI>(a, q, i);
-        const float4 ot = q.o_tmin[i], dt = q.d_tmax[i];
-        Ray3 r;
-        r.o = V(ot.x, ot.y, ot.z);
-        r.d = V(dt.x, dt.y, dt.z);
-        r.tmin = ot.w;
-        r.tmax = dt.w;
-        const gpr_hit h = trace_one(a, sv, r, st, true, REF_ORDER);
-        if (h.id == -1) {
-            const float4 wp = q.w_pdf[i];
-            fb_add(a.fb, q.pix[i] & 0x7fffffffu, C(wp.x, wp.y, wp.z));
-        }
-    });
+    uint32_t* cur = &a.ctrl->sh_cursor[bounce];
+    if constexpr (REFILL && !REF_ORDER) {
+        uint32_t ei = 0;
+        SceneView sv;
+        Trav t;
+        refill_loop(
+            cur, n, a.chunk,
+            [&](uint32_t i) {
+                ei = i;
+                sv = scene_of<MULTI>(a, q, i);
+                if (!trav_begin<true>(sv, queue_ray(q, i), st, t)) return false;
+                shadow_finish(a, q, i, t.hit);
+                return true;
+            },
+            [&]() {
+                const int rc = trav_step<true>(sv, st, t);
+                if (rc == kTravMore) return false;
+                if (rc == kTravOverflow) atomicOr(&a.ctrl->error, 1u);
+                shadow_finish(a, q, ei, t.hit);
+                return true;
+            });
+    } else {
+        for_each_claimed(cur, n, a.chunk, [&](uint32_t i) {
+            const SceneView sv = scene_of<MULTI>(a, q, i);
+            const gpr_hit h = trace_one(a, sv, queue_ray(q, i), st, true, REF_ORDER);
+            shadow_finish(a, q, i, h);
+        });
+    }
 }
 
 // ------------------------------------------------------------- shade ----
@@ -592,37 +697,43 @@ struct PassKernels {
     void (*shadow)(RenderArgs, uint32_t);
     int nt, sl, layout;
 };
-template <int NT, int SL>
+template <int NT, int SL, bool R>
 PassKernels pass_kernels() {
-    return {k_extend<true, false, true, NT, SL>, k_extend<false, false, true, NT, SL>,
-            k_shadow<false, true, NT, SL>, NT, SL, 0};
+    return {k_extend<true, false, true, NT, SL, R>, k_extend<false, false, true, NT, SL, R>,
+            k_shadow<false, true, NT, SL, R>, NT, SL, 0};
 }
 const PassKernels& pass_kernel_table(int idx) {
     static const PassKernels t[] = {
-        pass_kernels<64, 8>(),   pass_kernels<64, 16>(),  pass_kernels<64, 32>(),
-        pass_kernels<128, 8>(),  pass_kernels<128, 16>(), pass_kernels<128, 32>(),
-        pass_kernels<256, 8>(),  pass_kernels<256, 16>(), pass_kernels<256, 32>(),
-        pass_kernels<512, 8>(),  pass_kernels<512, 16>(), pass_kernels<512, 32>(),
+        pass_kernels<64, 8, false>(),   pass_kernels<64, 16, false>(),  pass_kernels<64, 32, false>(),
+        pass_kernels<128, 8, false>(),  pass_kernels<128, 16, false>(), pass_kernels<128, 32, false>(),
+        pass_kernels<256, 8, false>(),  pass_kernels<256, 16, false>(), pass_kernels<256, 32, false>(),
+        pass_kernels<512, 8, false>(),  pass_kernels<512, 16, false>(), pass_kernels<512, 32, false>(),
+        pass_kernels<64, 8, true>(),    pass_kernels<64, 16, true>(),   pass_kernels<64, 32, true>(),
+        pass_kernels<128, 8, true>(),   pass_kernels<128, 16, true>(),  pass_kernels<128, 32, true>(),
+        pass_kernels<256, 8, true>(),   pass_kernels<256, 16, true>(),  pass_kernels<256, 32, true>(),
+        pass_kernels<512, 8, true>(),   pass_kernels<512, 16, true>(),  pass_kernels<512, 32, true>(),
         {k_extend<true, true, true, 256, 16>, k_extend<false, true, true, 256, 16>, k_shadow<true, true, 256, 16>,
          256, 16, 1},
     };
     return t[idx];
 }
-constexpr int kPassKernelCount = 13;
+constexpr int kPassKernelCount = 25;
 int pass_kernel_index(const TraceCfg& c) {
-    if (c.layout == 1) return 12;
+    if (c.layout == 1) return c.refill ? -1 : 24;
     const int a = c.nt == 64 ? 0 : c.nt == 128 ? 1 : c.nt == 256 ? 2 : c.nt == 512 ? 3 : -1;
     const int b = c.sl == 8 ? 0 : c.sl == 16 ? 1 : c.sl == 32 ? 2 : -1;
-    return (a < 0 || b < 0) ? -1 : 3 * a + b;
+    return (a < 0 || b < 0) ? -1 : (c.refill ? 12 : 0) + 3 * a + b;
 }
 }  // namespace
 
 std::vector<TraceCfg> engine_tune_space() {
     std::vector<TraceCfg> v;
     for (int nt : {64, 128, 256, 512})
-        for (int sl : {8, 16, 32})
-            for (int chunk : {64, 128, 256, 512}) v.push_back(TraceCfg{nt, sl, chunk, 0});
-    v.push_back(TraceCfg{256, 16, 64, 1});
+        for (int sl : {8, 16, 32}) {
+            for (int chunk : {64, 128, 256, 512}) v.push_back(TraceCfg{nt, sl, chunk, 0, 0});
+            for (int chunk : {64, 128}) v.push_back(TraceCfg{nt, sl, chunk, 0, 1});
+        }
+    v.push_back(TraceCfg{256, 16, 64, 1, 0});
     return v;
 }
 

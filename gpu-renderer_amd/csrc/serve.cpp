@@ -100,11 +100,14 @@ int make_engine_job(const gpr_job& j, const std::vector<SceneView>& views, uint3
     return GPR_OK;
 }
 
-TraceCfg to_cfg(const gpr_trace_cfg& c) { return TraceCfg{(int)c.block, (int)c.stack, (int)c.chunk, (int)c.layout}; }
+TraceCfg to_cfg(const gpr_trace_cfg& c) {
+    return TraceCfg{(int)c.block, (int)c.stack, (int)c.chunk, (int)c.layout, c.refill ? 1 : 0};
+}
 
 bool cfg_valid(const TraceCfg& t) {
     bool ok = false;
-    for (const TraceCfg& s : engine_tune_space()) ok = ok || (s.nt == t.nt && s.sl == t.sl && s.layout == t.layout);
+    for (const TraceCfg& s : engine_tune_space())
+        ok = ok || (s.nt == t.nt && s.sl == t.sl && s.layout == t.layout && s.refill == t.refill);
     return ok && t.chunk >= 64 && t.chunk % 64 == 0 && t.chunk <= 4096;
 }
 
@@ -127,7 +130,8 @@ int tune_engine(Engine* e, const std::vector<EngineJob>& pass, float4* fb, hipSt
                              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             rc = fail(GPR_ERR_HIP, "tune: event timing");
         if (rc != GPR_OK) break;
-        res.push_back(gpr_trace_cfg{(uint32_t)c.nt, (uint32_t)c.sl, (uint32_t)c.chunk, (uint32_t)c.layout, 0, ms / 2});
+        res.push_back(gpr_trace_cfg{(uint32_t)c.nt, (uint32_t)c.sl, (uint32_t)c.chunk, (uint32_t)c.layout,
+                                    (uint32_t)c.refill, ms / 2});
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -155,7 +159,7 @@ struct gpr_batch {
     DevBuf fb;
     uint64_t fb_pixels = 0;
     bool fb_ready = false;
-    TraceCfg cfg{256, 16, 64, 0};
+    TraceCfg cfg{256, 16, 64, 0, 0};
     gpr_batch_stats stats{};
     ~gpr_batch() {
         if (eng) engine_destroy(eng);
@@ -373,7 +377,7 @@ struct gpr_server {
     DevBuf fb;
     PoolAlloc pool;
     hipStream_t st = nullptr, cst = nullptr;
-    TraceCfg cfg{256, 16, 64, 0};
+    TraceCfg cfg{256, 16, 64, 0, 0};
     std::mutex mu;
     std::condition_variable cv;       // loop thread: new work / stop
     std::condition_variable done_cv;  // waiters: a request finished

@@ -485,14 +485,14 @@ class Batch:
         self.jobs.append((j.width, j.height))
         return i.value
 
-    def tune(self, cap=64):
+    def tune(self, cap=128):
         out = (TraceCfg * cap)()
         n = C.c_uint32()
         check(lib.gpr_batch_tune(self.h, out, cap, C.byref(n)))
         return [out[k] for k in range(n.value)]
 
-    def set_cfg(self, block, stack, chunk, layout=0):
-        c = TraceCfg(block, stack, chunk, layout, 0, 0.0)
+    def set_cfg(self, block, stack, chunk, layout=0, refill=0):
+        c = TraceCfg(block, stack, chunk, layout, refill, 0.0)
         check(lib.gpr_batch_set_cfg(self.h, C.byref(c)))
 
     def run(self):
@@ -536,15 +536,15 @@ class Server:
         check(lib.gpr_server_add_scene(self.h, scene.h, C.byref(i)))
         return i.value
 
-    def tune(self, jobs, cap=64):
+    def tune(self, jobs, cap=128):
         arr = (Job * len(jobs))(*jobs)
         out = (TraceCfg * cap)()
         n = C.c_uint32()
         check(lib.gpr_server_tune(self.h, arr, len(jobs), out, cap, C.byref(n)))
         return [out[k] for k in range(n.value)]
 
-    def set_cfg(self, block, stack, chunk, layout=0):
-        c = TraceCfg(block, stack, chunk, layout, 0, 0.0)
+    def set_cfg(self, block, stack, chunk, layout=0, refill=0):
+        c = TraceCfg(block, stack, chunk, layout, refill, 0.0)
         check(lib.gpr_server_set_cfg(self.h, C.byref(c)))
 
     def submit(self, j):

@@ -89,10 +89,11 @@ def test_tuner_configs_agree(scenes):
         b.add_job(j)
     print("[serve] autotune", flush=True)
     res = b.tune()
-    assert len(res) == 4 * 3 * 4 + 1
+    assert len(res) == 4 * 3 * 6 + 1
     assert all(res[k].ms <= res[k + 1].ms for k in range(len(res) - 1))
     ref = None
-    for cfg in [(64, 8, 64, 0), (512, 32, 512, 0), (256, 16, 128, 1), (res[0].block, res[0].stack, res[0].chunk, res[0].layout)]:
+    for cfg in [(64, 8, 64, 0, 0), (512, 32, 512, 0, 0), (128, 16, 64, 0, 1), (512, 8, 128, 0, 1),
+                (256, 16, 128, 1, 0), (res[0].block, res[0].stack, res[0].chunk, res[0].layout, res[0].refill)]:
         b.set_cfg(*cfg)
         b.run()
         got = [b.read(k) for k in range(len(js))]
@@ -115,7 +116,7 @@ def test_server_alone_equals_mix_equals_plain(scenes):
     for s in scenes:
         srv.add_scene(s)
     res_t = srv.tune(js[:2])
-    assert len(res_t) == 49 and all(res_t[k].ms <= res_t[k + 1].ms for k in range(len(res_t) - 1))
+    assert len(res_t) == 73 and all(res_t[k].ms <= res_t[k + 1].ms for k in range(len(res_t) - 1))
     srv.set_cfg(64, 16, 128, 0)
     print("[serve] server alone", flush=True)
     alone, _ = srv.wait(srv.submit(js[1]))
