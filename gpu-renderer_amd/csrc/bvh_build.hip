@@ -195,16 +195,23 @@ __global__ __launch_bounds__(kB) void k_weights(const uint32_t* __restrict__ sk,
 // right child).  The second child to arrive at a parent forms its box.
 //
 // Everything one thread publishes for another (boxes, child ids, range
-// bounds) goes through device-scope atomic read-modify-writes, which are
-// performed at the device-coherent level; the writer waits for their
-// completion before it bumps the arrival counter and the reader only loads
-// (again by RMW) after seeing that counter.  No L2 write-back fence is needed
-// although the eight XCD L2 caches are not coherent with each other.
-GPR_D void pub(uint32_t* p, uint32_t v) {
-    (void)__hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// bounds) is written with device-scope relaxed atomic stores, which write
+// through to the device-coherent level; the writer waits for their completion
+// before it bumps the arrival counter (a device-scope RMW) and the reader
+// loads with device-scope atomic loads only after seeing that counter.  No L2
+// write-back fence is needed although the eight XCD L2 caches are not
+// coherent with each other.  Values read back are range-checked, so a
+// protocol failure raises the error word instead of indexing out of bounds.
+GPR_D void pub(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+GPR_D uint32_t get(uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+GPR_D void pub2(uint64_t* p, float a, float b) {
+    const uint64_t v = (uint64_t)__float_as_uint(a) | ((uint64_t)__float_as_uint(b) << 32);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-GPR_D uint32_t get(uint32_t* p) {
-    return __hip_atomic_fetch_or(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+GPR_D void get2(uint64_t* p, float& a, float& b) {
+    const uint64_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    a = __uint_as_float((uint32_t)v);
+    b = __uint_as_float((uint32_t)(v >> 32));
 }
 
 __global__ __launch_bounds__(kB) void k_tree(const float* __restrict__ pos, uint32_t n,
@@ -221,16 +228,14 @@ __global__ __launch_bounds__(kB) void k_tree(const float* __restrict__ pos, uint
     v3 lo, hi;
     tri_bounds(pos + 9ull * sidx[i], lo, hi);
     uint32_t node = (n - 1) + i;
-    uint32_t* bx = reinterpret_cast<uint32_t*>(box);
+    // a node's box is 24 B at 6*node floats: three 8-byte words
+    uint64_t* bx = reinterpret_cast<uint64_t*>(box);
     uint32_t l = i, r = i;
     for (int guard = 0; guard < kMaxDepth; ++guard) {
-        uint32_t* b = bx + 6ull * node;
-        pub(b + 0, __float_as_uint(lo.x));
-        pub(b + 1, __float_as_uint(lo.y));
-        pub(b + 2, __float_as_uint(lo.z));
-        pub(b + 3, __float_as_uint(hi.x));
-        pub(b + 4, __float_as_uint(hi.y));
-        pub(b + 5, __float_as_uint(hi.z));
+        uint64_t* b = bx + 3ull * node;
+        pub2(b + 0, lo.x, lo.y);
+        pub2(b + 1, lo.z, hi.x);
+        pub2(b + 2, hi.y, hi.z);
         if (l == 0 && r == n - 1) {
             *root = node;
             break;
@@ -254,9 +259,15 @@ __global__ __launch_bounds__(kB) void k_tree(const float* __restrict__ pos, uint
         if (as_left) r = get(&rhi[p]);
         else l = get(&rlo[p]);
         const uint32_t sib = get(&child[2 * p + (as_left ? 1 : 0)]);
-        uint32_t* sb = bx + 6ull * sib;
-        const v3 slo = V(__uint_as_float(get(sb + 0)), __uint_as_float(get(sb + 1)), __uint_as_float(get(sb + 2)));
-        const v3 shi = V(__uint_as_float(get(sb + 3)), __uint_as_float(get(sb + 4)), __uint_as_float(get(sb + 5)));
+        if (r >= n || l > p || (as_left && r <= p) || sib >= 2 * n - 1) {
+            atomicOr(root + 1, 1u);
+            break;
+        }
+        uint64_t* sb = bx + 3ull * sib;
+        v3 slo, shi;
+        get2(sb + 0, slo.x, slo.y);
+        get2(sb + 1, slo.z, shi.x);
+        get2(sb + 2, shi.y, shi.z);
         // Union(l_child, r_child) (aabb.h:37-46, bvh.h:182)
         if (as_left) {
             lo = vmin3(lo, slo);
@@ -502,6 +513,7 @@ int build_bvh(gpr_scene* s) {
         GPR_TRY(sort_pairs(keys, skeys, sidx, n, B.sort_scratch.p, B.sort_scratch.bytes, &B.sort_calls, st));
         GPR_HIP(hipEventRecord(ev[2], st));
         hipLaunchKernelGGL(k_fill_u32, dim3(std::min(1024u, div_up(n, kB))), dim3(kB), 0, st, arrive, 0u, (uint64_t)n);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kB), 0, st, root + 1, 0u, (uint64_t)1);
         hipLaunchKernelGGL(k_weights, dim3(div_up(n, kB)), dim3(kB), 0, st, skeys, n, w);
         hipLaunchKernelGGL(k_tree, dim3(div_up(n, kB)), dim3(kB), 0, st, pos, n, sidx, w, parent,
                            child, rlo, rhi, arrive, box, root);
@@ -518,9 +530,11 @@ int build_bvh(gpr_scene* s) {
         hipLaunchKernelGGL(k_tris, dim3(div_up(n, kB)), dim3(kB), 0, st, pos, n, sidx, tris);
         GPR_HIP(hipEventRecord(ev[4], st));
         GPR_HIP(hipGetLastError());
-        uint32_t nb4 = 0;
+        uint32_t nb4 = 0, tree_err = 0;
         GPR_HIP(hipMemcpyAsync(&nb4, total4, 4, hipMemcpyDeviceToHost, st));
+        GPR_HIP(hipMemcpyAsync(&tree_err, root + 1, 4, hipMemcpyDeviceToHost, st));
         GPR_HIP(hipStreamSynchronize(st));
+        if (tree_err) return fail(GPR_ERR_INVALID, "LBVH build: inconsistent node publication");
         B.nbvh4 = nb4;
         if (n > 1 && (nb4 == 0 || nb4 > n - 1)) return fail(GPR_ERR_INVALID, "LBVH build produced an invalid BVH4 node count");
         B.root4 = n == 1 ? ~(int32_t)0 : 0;
