@@ -47,7 +47,7 @@ def parse():
                     help="served requests (default: 48 in the raycast run, 256 with --workload serve)")
     ap.add_argument("--scenes", type=int, default=0, help="scenes of the batch / serve workloads (default 256 / 64)")
     ap.add_argument("--no-serve", action="store_true", help="raycast run without the served-stream segment")
-    ap.add_argument("--capacity", type=int, default=1 << 23, help="paths per pass of the served / batched engine")
+    ap.add_argument("--capacity", type=int, default=1 << 24, help="paths per pass of the served / batched engine")
     ap.add_argument("--trace-cfg", default="128,16,64,1",
                     help="block,stack,chunk,refill of the --only trace profiling run (the tuned configuration)")
     return ap.parse_args()
@@ -209,7 +209,7 @@ def run_batch(a, gpr, np_, torch):
     return out
 
 
-def serve_stream(a, gpr, np_, torch, n_req, n_scenes, verbose=True):
+def serve_stream(a, gpr, np_, torch, n_req, n_scenes, with_rmse=False):
     """Poisson request stream (lambda = 200/s) over device-resident scenes
     (sphere grids, 1e4..4e6 triangles log-uniform); each request a random
     scene, a camera on a sphere around it, resolution in {256^2, 512^2,
@@ -238,6 +238,7 @@ def serve_stream(a, gpr, np_, torch, n_req, n_scenes, verbose=True):
     tuned = srv.tune(reqs[: min(8, n_req)])
     tune_s = time.perf_counter() - t_tune
     tickets = [None] * n_req
+    k_rmse = next((k for k in range(n_req) if reqs[k].width == 256), None) if with_rmse else None
 
     def client():
         start = time.perf_counter()
@@ -255,7 +256,7 @@ def serve_stream(a, gpr, np_, torch, n_req, n_scenes, verbose=True):
     for k in range(n_req):
         while tickets[k] is None:
             time.sleep(0.0005)
-        served.append(srv.wait(tickets[k], want_image=(k < 4)))
+        served.append(srv.wait(tickets[k], want_image=(k < 4 or k == k_rmse)))
     th.join()
     served_s = time.perf_counter() - ts
     st = srv.stats()
@@ -279,6 +280,24 @@ def serve_stream(a, gpr, np_, torch, n_req, n_scenes, verbose=True):
                     for k in range(min(4, n_req)))
     lat = np.array([x[1] for x in served])
     paths = sum(j.width * j.height * j.spp for j in reqs)
+    rmse = None
+    if with_rmse:
+        # RMSE of a served image (mean of its spp samples, linear radiance, mean
+        # over pixels and channels) against a 4096-spp render of the same request
+        k = k_rmse
+        if k is not None:
+            j = reqs[k]
+            s = scenes[j.scene]
+            s.set_camera(np.array(j.c2w[:], np.float32), j.fov_deg, j.width, j.height)
+            r = gpr.Renderer(s, max_depth=j.max_depth, rng_mode=gpr.RNG_PCG, rr_start=j.rr_start, seed=j.seed + 1000,
+                             traversal=gpr.TRAVERSAL_FAST)
+            r.render(4096)
+            ref = r.read()[0] / 4096.0
+            r.free()
+            img = served[k][0] / float(j.spp)
+            rmse = {"request": k, "resolution": j.width, "spp": j.spp,
+                    "rmse_vs_4096spp": round(float(np.sqrt(np.mean((img - ref) ** 2))), 6),
+                    "reference_mean": round(float(ref.mean()), 6)}
     return {"renders_per_s": round(n_req / served_s, 3), "p50_ms": round(float(np.percentile(lat, 50)), 1),
             "p99_ms": round(float(np.percentile(lat, 99)), 1),
             "server_renders_per_s": round(st.renders_per_s, 3),
@@ -289,6 +308,7 @@ def serve_stream(a, gpr, np_, torch, n_req, n_scenes, verbose=True):
             "mpaths_per_s": round(paths / served_s / 1e6, 2),
             "mean_jobs_per_pass": round(st.mean_jobs_per_pass, 2), "passes": st.passes,
             "images_identical_to_plain": identical,
+            "rmse": rmse,
             "tuned": {"block": tuned[0].block, "stack": tuned[0].stack, "chunk": tuned[0].chunk,
                       "layout": tuned[0].layout, "tune_s": round(tune_s, 2)},
             "offered_load": "Poisson 200 requests/s",
@@ -297,7 +317,7 @@ def serve_stream(a, gpr, np_, torch, n_req, n_scenes, verbose=True):
 
 def run_serve(a, gpr, np_, torch):
     n_req = a.requests or 256
-    res = serve_stream(a, gpr, np_, torch, n_req, a.scenes or 64)
+    res = serve_stream(a, gpr, np_, torch, n_req, a.scenes or 64, with_rmse=True)
     out = {"metric": "served renders/s (Poisson request stream)", "value": res["renders_per_s"],
            "unit": "renders/s", "n_gpus": 1, "steps": n_req, "warmup": 0,
            "ms_per_step": round(1000.0 / max(res["renders_per_s"], 1e-9), 2), "higher_is_better": True,
