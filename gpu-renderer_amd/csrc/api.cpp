@@ -207,6 +207,33 @@ static int upload(DevBuf& b, const std::vector<T>& v) {
     return GPR_OK;
 }
 
+// Camera::Camera (camera.h:9-19): eye (0,2,7), direction (0,-0.2,-1), up
+// (0,1,0), 40-degree film, yaw / pitch of the direction.
+gpr_scene::gpr_scene() {
+    yaw = std::atan2(dir[0], dir[2]);
+    pitch = std::asin(dir[1]);
+    fovx = fovy = 40.f / 180 * 3.14159265358979323846f;
+}
+
+namespace {
+F3 f3mul(F3 a, float s) { return F3{a.x * s, a.y * s, a.z * s}; }
+F3 f3add(F3 a, F3 b) { return F3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+// film frame of the free camera (Camera::generateRay, camera.h:89-107)
+void free_frame(const gpr_scene* s, SceneView& v) {
+    const F3 d = f3norm(F3{s->dir[0], s->dir[1], s->dir[2]});
+    const F3 up{s->up[0], s->up[1], s->up[2]};
+    const F3 h = f3norm(f3cross(d, up));
+    const F3 vv = f3norm(f3cross(h, d));
+    const F3 eye{s->eye[0], s->eye[1], s->eye[2]};
+    const F3 mid = f3add(eye, d);
+    const F3 hs = f3mul(h, std::tan(s->fovx * 0.5f));
+    const F3 vs = f3mul(vv, std::tan(s->fovy * 0.5f));
+    const float* src[4] = {&eye.x, &mid.x, &hs.x, &vs.x};
+    float* dst[4] = {v.cam_eye, v.cam_mid, v.cam_h, v.cam_v};
+    for (int k = 0; k < 4; ++k) std::memcpy(dst[k], src[k], sizeof(float) * 3);
+}
+}  // namespace
+
 gpr::SceneView gpr_scene::view() const {
     SceneView v{};
     v.ntri = (uint32_t)mat.size();
@@ -231,6 +258,8 @@ gpr::SceneView gpr_scene::view() const {
     std::memcpy(v.c2w, c2w, sizeof(c2w));
     v.width = width;
     v.height = height;
+    v.cam_free = cam_free ? 1u : 0u;
+    if (cam_free) free_frame(this, v);
     return v;
 }
 
@@ -301,7 +330,25 @@ int gpr_scene_set_camera(gpr_scene* s, const float c2w[16], float fov_deg, uint3
     s->width = w;
     s->height = h;
     s->cam_set = true;
+    s->cam_free = false;
+    s->fovx = s->fovy = fov_deg / 180 * 3.14159265358979323846f;  // Camera::setup: make_float2(degToRad(fov))
     camera_raster_to_camera(w, h, fov_deg, s->r2c);
+    return GPR_OK;
+}
+
+int gpr_scene_set_camera_free(gpr_scene* s, const float eye[3], const float dir[3], const float up[3],
+                              float fov_deg, uint32_t w, uint32_t h) {
+    if (!s || !eye || !dir || !up || w == 0 || h == 0 || !(fov_deg > 0 && fov_deg < 180))
+        return fail(GPR_ERR_INVALID, "gpr_scene_set_camera_free: bad argument");
+    if (!(dir[0] != 0 || dir[1] != 0 || dir[2] != 0)) return fail(GPR_ERR_INVALID, "gpr_scene_set_camera_free: zero direction");
+    float ident[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    GPR_TRY(gpr_scene_set_camera(s, ident, fov_deg, w, h));
+    std::memcpy(s->eye, eye, sizeof(float) * 3);
+    std::memcpy(s->dir, dir, sizeof(float) * 3);
+    std::memcpy(s->up, up, sizeof(float) * 3);
+    s->yaw = std::atan2(dir[0], dir[2]);
+    s->pitch = std::asin(dir[1] < -1.f ? -1.f : (dir[1] > 1.f ? 1.f : dir[1]));
+    s->cam_free = true;
     return GPR_OK;
 }
 
@@ -404,6 +451,14 @@ int gpr_scene_get_info(const gpr_scene* s, gpr_scene_info* o) {
     std::memcpy(o->env, s->env, sizeof(o->env));
     o->bvh4_nodes = s->bvh.nbvh4;
     o->committed = s->committed ? 1u : 0u;
+    o->cam_free = s->cam_free ? 1u : 0u;
+    std::memcpy(o->eye, s->eye, sizeof(o->eye));
+    std::memcpy(o->dir, s->dir, sizeof(o->dir));
+    std::memcpy(o->up, s->up, sizeof(o->up));
+    o->fov_x = s->fovx;
+    o->fov_y = s->fovy;
+    o->yaw = s->yaw;
+    o->pitch = s->pitch;
     return GPR_OK;
 }
 

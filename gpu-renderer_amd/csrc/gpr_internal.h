@@ -99,6 +99,10 @@ struct SceneView {
     float r2c[16];
     float c2w[16];
     uint32_t width, height;
+    // free camera (camera.h:89-107): per pixel o = eye,
+    // d = normalize(mid + (2 px/W - 1) h + (2 py/H - 1) v - eye)
+    uint32_t cam_free;
+    float cam_eye[3], cam_mid[3], cam_h[3], cam_v[3];
 };
 
 struct BvhDev {
@@ -133,6 +137,11 @@ struct gpr_scene {
     float fov = 40.f;
     uint32_t width = 256, height = 256;
     bool cam_set = false;
+    // free camera state; defaults of Camera::Camera (camera.h:9-19)
+    bool cam_free = false;
+    float eye[3] = {0.f, 2.f, 7.f}, dir[3] = {0.f, -0.2f, -1.f}, up[3] = {0.f, 1.f, 0.f};
+    float yaw = 0.f, pitch = 0.f, fovx = 0.f, fovy = 0.f;  // set by the constructor below
+    gpr_scene();
     uint32_t has_env = 0;
     float env[3] = {0, 0, 0};
     // flattened host copies

@@ -486,7 +486,7 @@ struct Loader {
     F3 cur_light{0, 0, 0};
     std::map<std::string, gpr_material> named;
     uint32_t film_w = 256, film_h = 256;
-    bool cam = false;
+    bool cam = false, cam_free = false;
     float cam_c2w[16];
     float cam_fov = 30.f;
     int err = GPR_OK;
@@ -644,8 +644,7 @@ struct Loader {
             Xform c2w;
             auto it = mine.t.find("toWorld");
             if (it != mine.t.end()) c2w = it->second;
-            if (mine.getb("free", false))
-                return bad(GPR_ERR_UNSUPPORTED, "free-flying camera mode");
+            cam_free = mine.getb("free", false);  // GUI camera: the default eye / direction of Camera
             std::memcpy(cam_c2w, c2w.m.m, sizeof(cam_c2w));
             cam_fov = mine.getf("fov", 30.f);
             cam = true;
@@ -787,7 +786,10 @@ int load_xml(const std::string& path, gpr_scene* s) {
     L.scene = s;
     Props top;
     if (L.walk(*root, top) != GPR_OK) return L.err ? L.err : GPR_ERR_PARSE;
-    if (L.cam) GPR_TRY(gpr_scene_set_camera(s, L.cam_c2w, L.cam_fov, L.film_w, L.film_h));
+    if (L.cam && L.cam_free) {
+        const float eye[3] = {0.f, 2.f, 7.f}, dir[3] = {0.f, -0.2f, -1.f}, up[3] = {0.f, 1.f, 0.f};
+        GPR_TRY(gpr_scene_set_camera_free(s, eye, dir, up, L.cam_fov, L.film_w, L.film_h));
+    } else if (L.cam) GPR_TRY(gpr_scene_set_camera(s, L.cam_c2w, L.cam_fov, L.film_w, L.film_h));
     else {
         s->width = L.film_w;
         s->height = L.film_h;

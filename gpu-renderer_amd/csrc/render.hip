@@ -132,6 +132,15 @@ GPR_D void camera_ray(const float* m, const float* c, float px, float py, v3& o,
                 c[8] * n.x + c[9] * n.y + c[10] * n.z));
 }
 
+// Camera::generateRay (camera.h:89-107), free mode: film frame precomputed on the host.
+GPR_D void camera_ray_free(const SceneView& sv, float px, float py, v3& o, v3& d) {
+    const float sx = 2.f * (px / (float)sv.width) - 1.f;
+    const float sy = 2.f * (py / (float)sv.height) - 1.f;
+    const v3 p = load3(sv.cam_mid) + load3(sv.cam_h) * sx + load3(sv.cam_v) * sy;
+    o = load3(sv.cam_eye);
+    d = norm3(p - o);
+}
+
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_gen(RenderArgs a) {
     const uint32_t tid = blockIdx.x * 256 + threadIdx.x;
@@ -195,7 +204,8 @@ __global__ __launch_bounds__(256) void k_gen(RenderArgs a) {
             q.rng[tid] = st;
         }
         v3 o, d;
-        camera_ray(a.sv.r2c, a.sv.c2w, (float)x + jx, (float)y + jy, o, d);
+        if (a.sv.cam_free) camera_ray_free(a.sv, (float)x + jx, (float)y + jy, o, d);
+        else camera_ray(a.sv.r2c, a.sv.c2w, (float)x + jx, (float)y + jy, o, d);
         q.o_tmin[tid] = make_float4(o.x, o.y, o.z, 0.f);
         q.d_tmax[tid] = make_float4(d.x, d.y, d.z, 1e34f);
         q.w_pdf[tid] = make_float4(1.f, 1.f, 1.f, 0.f);
@@ -965,6 +975,7 @@ struct gpr_renderer {
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t graph = nullptr;
     SceneView baked{};  // scene view (camera included) the graph was captured with
+    bool reset_pending = false;  // PathTracer::m_reset: the next render starts a new accumulation
     ~gpr_renderer() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -1055,43 +1066,113 @@ int sample_graph(gpr_renderer* r, hipGraphExec_t* out) {
 
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+// Frame buffer, queues and traversal resources for the scene's current image size.
+int renderer_alloc(gpr_renderer* r) {
+    gpr_scene* s = r->scene;
+    const uint64_t npix = (uint64_t)s->width * s->height;
+    if (npix == 0 || npix >= (1ull << 31)) return fail(GPR_ERR_INVALID, "image size out of range");
+    r->npix = (uint32_t)npix;
+    r->ntiles = div_up(npix, kShadeBlock);
+    GPR_HIP(hipSetDevice(s->device));
+    GPR_TRY(r->fb.alloc(16ull * npix));
+    GPR_TRY(r->ctrl.alloc(sizeof(Ctrl)));
+    GPR_TRY(r->status.alloc(8ull * kMaxBounce * r->ntiles));
+    for (int k = 0; k < 3; ++k)
+        GPR_TRY(alloc_queue(r->qmem[k], r->q[k], r->npix, k < 2 ? &r->seeds[k] : nullptr, false));
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend<false, false, false, kTraceBlock, kStackLds>,
+                                                     kTraceBlock, 0) != hipSuccess)
+        return fail(GPR_ERR_HIP, "occupancy query failed");
+    r->grid_trace = std::max(1, cus * std::max(1, per_cu));
+    r->grid_trace = std::min<int>(r->grid_trace, (int)div_up(npix, kTraceBlock));
+    GPR_TRY(r->spill.alloc((size_t)r->grid_trace * kTraceBlock * kStackGlobal * sizeof(int2)));
+    return GPR_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int gpr_renderer_create(gpr_scene* s, const gpr_render_opts* o, gpr_renderer** out) {
     if (!s || !o || !out) return fail(GPR_ERR_INVALID, "gpr_renderer_create: null argument");
     if (!s->committed) return fail(GPR_ERR_INVALID, "gpr_renderer_create: scene not committed");
     if (o->max_depth > kMaxBounce) return fail(GPR_ERR_INVALID, "max_depth above 64");
     if (o->rng_mode > GPR_RNG_PCG || o->traversal > GPR_TRAVERSAL_REF)
         return fail(GPR_ERR_INVALID, "gpr_renderer_create: unknown mode");
-    const uint64_t npix = (uint64_t)s->width * s->height;
-    if (npix == 0 || npix >= (1ull << 31)) return fail(GPR_ERR_INVALID, "image size out of range");
     gpr_renderer* r = new (std::nothrow) gpr_renderer();
     if (!r) return fail(GPR_ERR_OOM, "gpr_renderer_create: host allocation failed");
     r->scene = s;
     r->opts = *o;
-    r->npix = (uint32_t)npix;
-    r->ntiles = div_up(npix, kShadeBlock);
-    GPR_HIP(hipSetDevice(s->device));
-    int rc = GPR_OK;
-    if (rc == GPR_OK) rc = r->fb.alloc(16ull * npix);
-    if (rc == GPR_OK) rc = r->ctrl.alloc(sizeof(Ctrl));
-    if (rc == GPR_OK) rc = r->status.alloc(8ull * kMaxBounce * r->ntiles);
-    for (int k = 0; k < 3 && rc == GPR_OK; ++k)
-        rc = alloc_queue(r->qmem[k], r->q[k], r->npix, k < 2 ? &r->seeds[k] : nullptr, false);
-    if (rc == GPR_OK) {
-        int cus = 0, per_cu = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, k_extend<false, false, false, kTraceBlock, kStackLds>, kTraceBlock, 0) != hipSuccess)
-            rc = fail(GPR_ERR_HIP, "occupancy query failed");
-        r->grid_trace = std::max(1, cus * std::max(1, per_cu));
-        r->grid_trace = std::min<int>(r->grid_trace, (int)div_up(npix, kTraceBlock));
-    }
-    if (rc == GPR_OK) rc = r->spill.alloc((size_t)r->grid_trace * kTraceBlock * kStackGlobal * sizeof(int2));
+    int rc = renderer_alloc(r);
     if (rc == GPR_OK) rc = gpr_renderer_reset(r);
     if (rc != GPR_OK) {
         delete r;
         return rc;
     }
     *out = r;
+    return GPR_OK;
+}
+
+int gpr_renderer_zoom(gpr_renderer* r, float d) {
+    if (!r) return fail(GPR_ERR_INVALID, "gpr_renderer_zoom: null renderer");
+    float* e = r->scene->eye;
+    const float* v = r->scene->dir;
+    for (int k = 0; k < 3; ++k) e[k] = e[k] + v[k] * d;  // m_eye += m_dir * d
+    r->reset_pending = true;
+    return GPR_OK;
+}
+
+int gpr_renderer_translate(gpr_renderer* r, float x, float y) {
+    if (!r) return fail(GPR_ERR_INVALID, "gpr_renderer_translate: null renderer");
+    gpr_scene* s = r->scene;
+    const v3 d = norm3(V(s->dir[0], s->dir[1], s->dir[2]));
+    const v3 h = norm3(cross3(d, V(s->up[0], s->up[1], s->up[2])));
+    const v3 vv = norm3(cross3(h, d));
+    const v3 m = h * x + vv * y;
+    s->eye[0] = s->eye[0] + m.x;
+    s->eye[1] = s->eye[1] + m.y;
+    s->eye[2] = s->eye[2] + m.z;
+    r->reset_pending = true;
+    return GPR_OK;
+}
+
+int gpr_renderer_rotate(gpr_renderer* r, float yaw, float pitch) {
+    if (!r) return fail(GPR_ERR_INVALID, "gpr_renderer_rotate: null renderer");
+    gpr_scene* s = r->scene;
+    s->yaw += yaw;
+    s->pitch += pitch;
+    s->dir[0] = std::sin(s->yaw) * std::cos(s->pitch);
+    s->dir[1] = std::sin(s->pitch);
+    s->dir[2] = std::cos(s->yaw) * std::cos(s->pitch);
+    r->reset_pending = true;
+    return GPR_OK;
+}
+
+int gpr_renderer_resize(gpr_renderer* r, uint32_t w, uint32_t h) {
+    if (!r || w == 0 || h == 0) return fail(GPR_ERR_INVALID, "gpr_renderer_resize: bad argument");
+    gpr_scene* s = r->scene;
+    GPR_HIP(hipStreamSynchronize(stream()));
+    s->width = w;
+    s->height = h;
+    if (!s->cam_free) camera_raster_to_camera(w, h, s->fov, s->r2c);
+    // Camera::updateAspectRation: fov.y = atan(tan(fov.x * 0.5) / aspect) * 2 (in double)
+    const float aspect = (float)w / (float)h;
+    s->fovy = (float)(std::atan(std::tan(s->fovx * 0.5) / aspect) * 2.0);
+    if (r->graph) {
+        (void)hipGraphExecDestroy(r->graph);
+        r->graph = nullptr;
+    }
+    GPR_TRY(renderer_alloc(r));
+    return gpr_renderer_reset(r);
+}
+
+int gpr_renderer_get_resolution(const gpr_renderer* r, uint32_t* w, uint32_t* h) {
+    if (!r || !w || !h) return fail(GPR_ERR_INVALID, "gpr_renderer_get_resolution: null argument");
+    *w = r->scene->width;
+    *h = r->scene->height;
     return GPR_OK;
 }
 
@@ -1106,12 +1187,16 @@ int gpr_renderer_reset(gpr_renderer* r) {
         r->graph = nullptr;
     }
     r->spp_done = 0;
+    r->reset_pending = false;
     return GPR_OK;
 }
 
 int gpr_render(gpr_renderer* r, uint32_t spp) {
     if (!r) return fail(GPR_ERR_INVALID, "gpr_render: null renderer");
     if (r->scene->bvh.n == 0) return fail(GPR_ERR_UNSUPPORTED, "gpr_render: scene has no triangles");
+    if ((uint64_t)r->scene->width * r->scene->height != r->npix)
+        return fail(GPR_ERR_INVALID, "gpr_render: the scene's image size changed; use gpr_renderer_resize");
+    if (r->reset_pending) GPR_TRY(gpr_renderer_reset(r));
     hipStream_t st = stream();
     hipGraphExec_t g = nullptr;
     GPR_TRY(sample_graph(r, &g));

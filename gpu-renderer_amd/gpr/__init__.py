@@ -44,7 +44,10 @@ class SceneInfo(C.Structure):
     _fields_ = [("ntri", C.c_uint32), ("nlights", C.c_uint32), ("nmat", C.c_uint32),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("fov_deg", C.c_float),
                 ("c2w", C.c_float * 16), ("has_env", C.c_uint32), ("env", C.c_float * 3),
-                ("bvh4_nodes", C.c_uint32), ("committed", C.c_uint32)]
+                ("bvh4_nodes", C.c_uint32), ("committed", C.c_uint32),
+                ("cam_free", C.c_uint32), ("eye", C.c_float * 3), ("dir", C.c_float * 3),
+                ("up", C.c_float * 3), ("fov_x", C.c_float), ("fov_y", C.c_float),
+                ("yaw", C.c_float), ("pitch", C.c_float)]
 
 
 class RenderOpts(C.Structure):
@@ -117,6 +120,12 @@ _SIG = {
                                      C.c_uint32, C.POINTER(Material)]),
     "gpr_scene_set_camera": (C.c_int, [_P, _P, C.c_float, C.c_uint32, C.c_uint32]),
     "gpr_scene_set_camera_lookat": (C.c_int, [_P, _P, _P, _P, C.c_float, C.c_uint32, C.c_uint32]),
+    "gpr_scene_set_camera_free": (C.c_int, [_P, _P, _P, _P, C.c_float, C.c_uint32, C.c_uint32]),
+    "gpr_renderer_zoom": (C.c_int, [_P, C.c_float]),
+    "gpr_renderer_translate": (C.c_int, [_P, C.c_float, C.c_float]),
+    "gpr_renderer_rotate": (C.c_int, [_P, C.c_float, C.c_float]),
+    "gpr_renderer_resize": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "gpr_renderer_get_resolution": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "gpr_scene_set_env": (C.c_int, [_P, _P]),
     "gpr_scene_commit": (C.c_int, [_P]),
     "gpr_scene_free": (C.c_int, [_P]),
@@ -298,6 +307,10 @@ class Scene:
         m = np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(16))
         check(lib.gpr_scene_set_camera(self.h, _ptr(m), fov, w, h))
 
+    def set_camera_free(self, eye, direction, up, fov, w, h):
+        e, d, u = (np.ascontiguousarray(np.asarray(v, np.float32)) for v in (eye, direction, up))
+        check(lib.gpr_scene_set_camera_free(self.h, _ptr(e), _ptr(d), _ptr(u), fov, w, h))
+
     def set_camera_lookat(self, target, origin, up, fov, w, h):
         t = np.asarray(target, np.float32)
         o = np.asarray(origin, np.float32)
@@ -405,6 +418,24 @@ class Renderer:
 
     def render(self, spp):
         check(lib.gpr_render(self.h, spp))
+
+    def zoom(self, d):
+        check(lib.gpr_renderer_zoom(self.h, d))
+
+    def translate(self, x, y):
+        check(lib.gpr_renderer_translate(self.h, x, y))
+
+    def rotate(self, yaw, pitch):
+        check(lib.gpr_renderer_rotate(self.h, yaw, pitch))
+
+    def resize(self, w, h):
+        check(lib.gpr_renderer_resize(self.h, w, h))
+        self.w, self.h_px = w, h
+
+    def resolution(self):
+        w, h = C.c_uint32(), C.c_uint32()
+        check(lib.gpr_renderer_get_resolution(self.h, C.byref(w), C.byref(h)))
+        return w.value, h.value
 
     def read(self):
         n = self.w * self.h_px

@@ -102,6 +102,12 @@ int gpr_scene_add_mesh(gpr_scene* s, const float* p, uint32_t np, const float* n
  * the row-major cameraToWorld matrix. */
 int gpr_scene_set_camera(gpr_scene* s, const float c2w[16], float fov_deg, uint32_t width,
                          uint32_t height);
+/* Free (GUI) camera: Camera::setup with free = true and the eye / direction /
+ * up state of Camera (camera.h:9-19, 89-107); rays through eye + dir with
+ * tan(fov/2)-scaled film axes.  fov_deg sets both film axes, as setup does;
+ * gpr_renderer_resize rescales the vertical one. */
+int gpr_scene_set_camera_free(gpr_scene* s, const float eye[3], const float dir[3], const float up[3],
+                              float fov_deg, uint32_t width, uint32_t height);
 /* LookAt (src/renderer/transform.cpp:220-248) + Camera::setup. */
 int gpr_scene_set_camera_lookat(gpr_scene* s, const float target[3], const float origin[3],
                                 const float up[3], float fov_deg, uint32_t width, uint32_t height);
@@ -127,6 +133,10 @@ typedef struct {
     float env[3];
     uint32_t bvh4_nodes; /* packed 128-B traversal nodes */
     uint32_t committed;
+    uint32_t cam_free;   /* 1 = free camera (eye / dir / up below), 0 = cameraToWorld */
+    float eye[3], dir[3], up[3];
+    float fov_x, fov_y;  /* radians (film axes of the free camera) */
+    float yaw, pitch;    /* free-camera orientation state (Camera::rotate) */
 } gpr_scene_info;
 int gpr_scene_get_info(const gpr_scene* s, gpr_scene_info* out);
 /* Flattened per-triangle data in triangle-id order (what the kernels read):
@@ -228,6 +238,17 @@ int gpr_renderer_read(gpr_renderer* r, float* rgb_sum, uint32_t* sample_num);
 int gpr_renderer_write_png(gpr_renderer* r, const char* path);
 /* PathTracer::reset (pathtracer.cu:195-200). */
 int gpr_renderer_reset(gpr_renderer* r);
+/* Interactive camera and frame operations of PathTracer (pathtracer.cu:177-209):
+ * they change the scene's free camera (Camera::zoom / translate / rotate,
+ * camera.h:40-63) or the image size, and the next gpr_render starts a new
+ * accumulation, as the reference's m_reset does. */
+int gpr_renderer_zoom(gpr_renderer* r, float d);
+int gpr_renderer_translate(gpr_renderer* r, float x, float y);
+int gpr_renderer_rotate(gpr_renderer* r, float yaw, float pitch);
+/* PathTracer::resize + Camera::updateAspectRation (camera.h:65-84): reallocates
+ * the frame buffer and queues. */
+int gpr_renderer_resize(gpr_renderer* r, uint32_t width, uint32_t height);
+int gpr_renderer_get_resolution(const gpr_renderer* r, uint32_t* width, uint32_t* height);
 /* Queue sizes of the last sample: in/shadow/scatter per bounce (up to cap
  * bounces); the reference's m_sum_bounce (pathtracer.cu:96). */
 int gpr_renderer_queue_sizes(gpr_renderer* r, uint32_t* shadow, uint32_t* scatter, uint32_t cap);

@@ -484,9 +484,21 @@ inline f3 xf_vector(const M4& m, f3 v) {
 // Camera::setup (camera.h:21-37), non-free (Mitsuba raster) mode.
 struct Cam {
     M4 r2c, c2w;
+    bool free_mode;
+    f3 eye, dir, up;
+    float fovx, fovy;
+    float w, h;
 };
 Cam make_camera(const orc_scene* s) {
     Cam c;
+    c.free_mode = s->cam_free != 0;
+    c.eye = mk(s->eye[0], s->eye[1], s->eye[2]);
+    c.dir = mk(s->dir[0], s->dir[1], s->dir[2]);
+    c.up = mk(s->up[0], s->up[1], s->up[2]);
+    c.fovx = s->fov_x;
+    c.fovy = s->fov_y;
+    c.w = (float)s->width;
+    c.h = (float)s->height;
     std::memcpy(c.c2w.m, s->c2w, sizeof(float) * 16);
     float fov = s->fov_deg / 180 * kPi;  // degToRad (fwd.h:25-28)
     float aspect = float(s->width) / s->height;
@@ -497,8 +509,21 @@ Cam make_camera(const orc_scene* s) {
     c.r2c = c2r.inv;  // Inverse(Transform) swaps m and mInv (transform.h:137-139)
     return c;
 }
-// Camera::generateRay (camera.h:86-119), non-free mode.
+// Camera::generateRay (camera.h:86-119).
 inline void gen_ray(const Cam& c, float px, float py, f3* o, f3* d) {
+    if (c.free_mode) {
+        // free mode (camera.h:89-107)
+        f3 dir = normalize(c.dir);
+        f3 horizontal_axis = normalize(cross(dir, c.up));
+        f3 vertical_axis = normalize(cross(horizontal_axis, dir));
+        f3 middle = c.eye + dir;
+        f3 horizontal = horizontal_axis * std::tan(c.fovx * 0.5f);
+        f3 vertical = vertical_axis * std::tan(c.fovy * 0.5f);
+        f3 film = middle + horizontal * (2.f * (px / c.w) - 1.f) + vertical * (2.f * (py / c.h) - 1.f);
+        *o = c.eye;
+        *d = normalize(film - c.eye);
+        return;
+    }
     f3 pc = xf_point(c.r2c, mk(px, py, 0));
     *o = xf_point(c.c2w, mk(0.f, 0.f, 0.f));
     *d = normalize(xf_vector(c.c2w, normalize(pc)));

@@ -62,6 +62,10 @@ typedef struct {
     uint32_t width, height;
     uint32_t has_env;       /* constant environment light */
     float env[3];
+    /* free camera (camera.h:89-107): eye, direction, up, film half-angles (radians) */
+    uint32_t cam_free;
+    float eye[3], dir[3], up[3];
+    float fov_x, fov_y;
 } orc_scene;
 
 typedef struct {

@@ -31,7 +31,9 @@ class Scene(C.Structure):
     _fields_ = [("ntri", C.c_uint32), ("pos", P), ("nrm", P), ("uv", P), ("flags", P), ("mat", P),
                 ("mats", P), ("nmat", C.c_uint32), ("c2w", C.c_float * 16), ("c2w_inv", C.c_float * 16),
                 ("fov_deg", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32),
-                ("has_env", C.c_uint32), ("env", C.c_float * 3)]
+                ("has_env", C.c_uint32), ("env", C.c_float * 3),
+                ("cam_free", C.c_uint32), ("eye", C.c_float * 3), ("dir", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_x", C.c_float), ("fov_y", C.c_float)]
 
 
 class RenderOpts(C.Structure):
@@ -145,8 +147,9 @@ def brute(pos, rays, any_hit=False, fp64=False, nthreads=NTHREADS):
     return hits, t64
 
 
-def make_scene(flat, c2w, fov, w, h, env=None):
-    """orc_scene from the flat arrays of gpr.Scene.flat(); keeps the arrays alive."""
+def make_scene(flat, c2w, fov, w, h, env=None, info=None):
+    """orc_scene from the flat arrays of gpr.Scene.flat(); keeps the arrays alive.
+    With `info` (gpr_scene_info) the free-camera state is taken over as well."""
     s = Scene()
     keep = {}
     for k in ("pos", "nrm", "uv", "flags", "mat"):
@@ -162,6 +165,12 @@ def make_scene(flat, c2w, fov, w, h, env=None):
     if env is not None:
         s.has_env = 1
         s.env[:] = env
+    if info is not None and info.cam_free:
+        s.cam_free = 1
+        s.eye[:] = list(info.eye)
+        s.dir[:] = list(info.dir)
+        s.up[:] = list(info.up)
+        s.fov_x, s.fov_y = info.fov_x, info.fov_y
     s._keep = keep
     return s
 

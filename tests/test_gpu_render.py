@@ -20,7 +20,7 @@ def oracle_render(scene, spp, depth, rng_mode, seed, rr_start=3):
     f = scene.flat()
     info = f["info"]
     osc = O.make_scene(f, np.array(info.c2w[:], np.float32), info.fov_deg, info.width, info.height,
-                       env=(list(info.env) if info.has_env else None))
+                       env=(list(info.env) if info.has_env else None), info=info)
     nodes, _, _ = O.lbvh(f["pos"])
     rgb, cnt = O.render(osc, nodes, spp, depth, rng_mode, seed=seed, rr_start=rr_start)
     return rgb, cnt
@@ -103,6 +103,40 @@ def test_env_light_and_reset():
     r.render(2)
     b, c2 = r.read()
     assert np.all(c2 == 2) and np.array_equal(a, b)  # reset restarts the sample sequence
+    r.free()
+
+
+def test_free_camera_and_interactive_ops_bit_exact():
+    # The reference's GUI scene (scene/CBox/cbox(mesh).xml, <bool name="free" value="true"/>):
+    # the free camera of Camera with its default eye / direction, then the
+    # PathTracer camera operations and a resize, each against the oracle.
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cbox", "cbox_mesh.xml")
+    s = gpr.Scene.load(path)
+    info = s.info()
+    assert info.cam_free == 1 and (info.width, info.height) == (800, 600)
+    np.testing.assert_array_equal(np.array(info.eye, np.float32), np.float32([0, 2, 7]))
+    r = gpr.Renderer(s, max_depth=3, rng_mode=gpr.RNG_PCG, rr_start=2, seed=4, traversal=gpr.TRAVERSAL_REF)
+    r.resize(160, 120)
+    assert r.resolution() == (160, 120)
+    info = s.info()
+    # updateAspectRation: fov.y = atan(tan(fov.x / 2) / aspect) * 2
+    assert abs(info.fov_y - np.arctan(np.tan(info.fov_x * 0.5) / (160 / 120)) * 2) < 1e-6
+    steps = [("resize", ()), ("zoom", (0.7,)), ("translate", (0.1, -0.2)), ("rotate", (0.05, -0.03))]
+    for name, args in steps:
+        if name != "resize":
+            getattr(r, name)(*args)
+        r.render(2)
+        g, _ = r.read()
+        o, _ = oracle_render(s, 2, 3, gpr.RNG_PCG, 4, rr_start=2)
+        print(f"[render] free camera after {name}", flush=True)
+        bad = np.nonzero(np.any(g.view(np.uint32) != o.view(np.uint32), axis=1))[0]
+        assert len(bad) == 0, (name, len(bad))
+        assert g.sum() > 0
+    info = s.info()
+    # Camera::rotate: direction from the accumulated yaw / pitch
+    want = [np.sin(info.yaw) * np.cos(info.pitch), np.sin(info.pitch), np.cos(info.yaw) * np.cos(info.pitch)]
+    np.testing.assert_allclose(np.array(info.dir), want, atol=1e-6)
     r.free()
 
 

@@ -150,11 +150,19 @@ def test_cbox_sphere_variant_uv_spheres():
     np.testing.assert_array_equal(f["mats"][-1]["emission"], np.float32([40] * 3))
 
 
-def test_cbox_free_camera_is_reported():
-    with pytest.raises(gpr.GprError) as e:
-        gpr.Scene.parse(os.path.join(CBOX, "cbox_mesh.xml"))
-    assert e.value.code == gpr.ERR_UNSUPPORTED
-    assert "free" in str(e.value)
+def test_cbox_free_camera_defaults():
+    # cbox(mesh).xml sets <bool name="free" value="true"/>: the GUI camera of
+    # Camera::Camera (eye (0,2,7), direction (0,-0.2,-1), up (0,1,0)) with the
+    # sensor's fov on both film axes (camera.h:9-37)
+    s = gpr.Scene.parse(os.path.join(CBOX, "cbox_mesh.xml"))
+    info = s.info()
+    assert info.cam_free == 1
+    np.testing.assert_array_equal(np.array(info.eye, np.float32), np.float32([0, 2, 7]))
+    np.testing.assert_array_equal(np.array(info.dir, np.float32), np.float32([0, -0.2, -1]))
+    np.testing.assert_array_equal(np.array(info.up, np.float32), np.float32([0, 1, 0]))
+    fov = np.float32(27.7856) / np.float32(180) * np.float32(np.pi)
+    assert info.fov_x == info.fov_y and abs(info.fov_x - fov) < 1e-7
+    assert abs(info.yaw - np.arctan2(0, -1)) < 1e-6 and abs(info.pitch - np.arcsin(-0.2)) < 1e-6
 
 
 # ------------------------------------------------------------ XML subset ----
