@@ -98,6 +98,16 @@ M4 m_inverse(const M4& a) {
 struct Xf {
     M4 m, inv;
 };
+// Inverse of a row-major 4x4; false when it is not finite (singular input).
+bool invert4(const float a[16], float out[16]) {
+    M4 m;
+    std::memcpy(m.m, a, sizeof(m.m));
+    const M4 r = m_inverse(m);
+    std::memcpy(out, r.m, sizeof(r.m));
+    for (int k = 0; k < 16; ++k)
+        if (!std::isfinite(out[k])) return false;
+    return true;
+}
 Xf x_mul(const Xf& a, const Xf& b) { return Xf{m_mul(a.m, b.m), m_mul(b.inv, a.inv)}; }
 Xf x_scale(float x, float y, float z) {
     Xf t{m_ident(), m_ident()};
@@ -260,6 +270,13 @@ gpr::SceneView gpr_scene::view() const {
     v.height = height;
     v.cam_free = cam_free ? 1u : 0u;
     if (cam_free) free_frame(this, v);
+    v.uv = d_uv.as<float>();
+    v.tex = d_tex.as<gpr_texture>();
+    v.texels = d_texels.as<float4>();
+    v.ntex = (uint32_t)textures.size();
+    v.env_tex = has_env ? env_tex : 0u;
+    std::memcpy(v.env_w2o, env_w2o, sizeof(env_w2o));
+    std::memcpy(v.env_center, env_center, sizeof(env_center));
     return v;
 }
 
@@ -298,6 +315,9 @@ int gpr_scene_add_mesh(gpr_scene* s, const float* p, uint32_t np, const float* n
     if (!s || !mat || (ntri && (!p || !corners))) return fail(GPR_ERR_INVALID, "gpr_scene_add_mesh: null argument");
     if (s->committed) return fail(GPR_ERR_INVALID, "gpr_scene_add_mesh: scene already committed");
     if (mat->type > GPR_MAT_ROUGH_PLASTIC) return fail(GPR_ERR_UNSUPPORTED, "unknown material type");
+    for (int k = 0; k < GPR_TEX_SLOTS; ++k)
+        if (mat->tex[k] > s->textures.size())
+            return fail(GPR_ERR_INVALID, "material texture handle " + std::to_string(mat->tex[k]) + " not in the scene");
     for (uint64_t t = 0; t < ntri; ++t) {
         const int32_t* c = corners + 9 * t;
         const bool hn = c[2] != -1, huv = c[1] != -1;
@@ -363,7 +383,67 @@ int gpr_scene_set_camera_lookat(gpr_scene* s, const float target[3], const float
 int gpr_scene_set_env(gpr_scene* s, const float rgb[3]) {
     if (!s || !rgb) return fail(GPR_ERR_INVALID, "gpr_scene_set_env: null argument");
     s->has_env = 1;
+    s->env_tex = 0;
     std::memcpy(s->env, rgb, sizeof(float) * 3);
+    return GPR_OK;
+}
+
+int gpr_scene_add_texture(gpr_scene* s, const float* rgb, uint32_t w, uint32_t h, const float uv_offset[2],
+                          const float uv_scale[2], uint32_t* out_id) {
+    if (!s || !rgb || !out_id || w == 0 || h == 0) return fail(GPR_ERR_INVALID, "gpr_scene_add_texture: bad argument");
+    if (s->committed) return fail(GPR_ERR_INVALID, "gpr_scene_add_texture: scene already committed");
+    const uint64_t n = (uint64_t)w * h, off = s->texels.size() / 4;
+    if (off + n >= (1ull << 31)) return fail(GPR_ERR_UNSUPPORTED, "texel pool above 2^31 texels");
+    gpr_texture t{};
+    t.offset = (uint32_t)off;
+    t.width = w;
+    t.height = h;
+    t.uv_offset[0] = uv_offset ? uv_offset[0] : 0.f;
+    t.uv_offset[1] = uv_offset ? uv_offset[1] : 0.f;
+    t.uv_scale[0] = uv_scale ? uv_scale[0] : 1.f;
+    t.uv_scale[1] = uv_scale ? uv_scale[1] : 1.f;
+    s->texels.resize(4 * (off + n), 0.f);
+    for (uint64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) s->texels[4 * (off + i) + c] = rgb[3 * i + c];
+    s->textures.push_back(t);
+    *out_id = (uint32_t)s->textures.size();
+    return GPR_OK;
+}
+
+int gpr_scene_add_checkerboard(gpr_scene* s, const float c0[3], const float c1[3], const float uv_offset[2],
+                               const float uv_scale[2], uint32_t* out_id) {
+    if (!c0 || !c1) return fail(GPR_ERR_INVALID, "gpr_scene_add_checkerboard: null colour");
+    float px[12];
+    for (int c = 0; c < 3; ++c) {
+        px[c] = px[9 + c] = c0[c];
+        px[3 + c] = px[6 + c] = c1[c];
+    }
+    return gpr_scene_add_texture(s, px, 2, 2, uv_offset, uv_scale, out_id);
+}
+
+int gpr_scene_set_env_map(gpr_scene* s, uint32_t tex_id, const float o2w[16]) {
+    if (!s || tex_id == 0 || tex_id > s->textures.size())
+        return fail(GPR_ERR_INVALID, "gpr_scene_set_env_map: bad scene or texture handle");
+    // worldToObject = Inverse(o2w) (environment.cpp:11-15); transformVector
+    // reads its upper-left 3x3 (transform.cpp:177-183)
+    float inv[16];
+    if (o2w) {
+        if (!invert4(o2w, inv)) return fail(GPR_ERR_INVALID, "gpr_scene_set_env_map: singular toWorld");
+    } else {
+        const float id[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        std::memcpy(inv, id, sizeof(id));
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) s->env_w2o[3 * r + c] = inv[4 * r + c];
+    s->has_env = 1;
+    s->env_tex = tex_id;
+    return GPR_OK;
+}
+
+int gpr_scene_export_textures(const gpr_scene* s, gpr_texture* recs, float* texels) {
+    if (!s) return fail(GPR_ERR_INVALID, "gpr_scene_export_textures: null scene");
+    if (recs && !s->textures.empty()) std::memcpy(recs, s->textures.data(), s->textures.size() * sizeof(gpr_texture));
+    if (texels && !s->texels.empty()) std::memcpy(texels, s->texels.data(), s->texels.size() * sizeof(float));
     return GPR_OK;
 }
 
@@ -383,6 +463,7 @@ int gpr_scene_flatten(gpr_scene* s) {
     s->mat.assign(total, 0);
     s->mats.clear();
     s->lights.clear();
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     uint64_t t = 0;
     for (size_t mi = 0; mi < s->meshes.size(); ++mi) {
         const Mesh& m = s->meshes[mi];
@@ -401,10 +482,17 @@ int gpr_scene_flatten(gpr_scene* s) {
             }
             s->flags[t] = fl;
             s->mat[t] = (uint32_t)mi;
+            for (int k = 0; k < 9; ++k) {
+                const float x = s->pos[9 * t + k];
+                lo[k % 3] = std::fmin(lo[k % 3], x);
+                hi[k % 3] = std::fmax(hi[k % 3], x);
+            }
             const float* e = m.mat.emission;
             if (!(e[0] == 0 && e[1] == 0 && e[2] == 0)) s->lights.push_back((uint32_t)t);
         }
     }
+    // world box centroid (scene.cpp:26-27, AABB::centroid aabb.h:23)
+    for (int a = 0; a < 3; ++a) s->env_center[a] = total ? (lo[a] + hi[a]) * 0.5f : 0.f;
     s->flattened = true;
     return GPR_OK;
 }
@@ -420,6 +508,11 @@ int gpr_scene_commit(gpr_scene* s) {
     GPR_TRY(upload(s->d_mat, s->mat));
     GPR_TRY(upload(s->d_mats, s->mats));
     GPR_TRY(upload(s->d_lights, s->lights));
+    if (!s->textures.empty()) {
+        GPR_TRY(upload(s->d_uv, s->uv));
+        GPR_TRY(upload(s->d_tex, s->textures));
+        GPR_TRY(upload(s->d_texels, s->texels));
+    }
     GPR_TRY(s->d_stats.alloc(256));
     if (!s->cam_set) camera_raster_to_camera(s->width, s->height, s->fov, s->r2c);
     GPR_TRY(build_bvh(s));
@@ -459,6 +552,11 @@ int gpr_scene_get_info(const gpr_scene* s, gpr_scene_info* o) {
     o->fov_y = s->fovy;
     o->yaw = s->yaw;
     o->pitch = s->pitch;
+    o->ntex = (uint32_t)s->textures.size();
+    o->ntexel = (uint32_t)(s->texels.size() / 4);
+    o->env_tex = s->has_env ? s->env_tex : 0u;
+    std::memcpy(o->env_w2o, s->env_w2o, sizeof(o->env_w2o));
+    std::memcpy(o->env_center, s->env_center, sizeof(o->env_center));
     return GPR_OK;
 }
 

@@ -103,6 +103,16 @@ struct SceneView {
     // d = normalize(mid + (2 px/W - 1) h + (2 py/H - 1) v - eye)
     uint32_t cam_free;
     float cam_eye[3], cam_mid[3], cam_h[3], cam_v[3];
+    // textures (texture.h:37-52): per-triangle corner uvs (read where flags&2
+    // and the material has a texture), texture table, RGBA texel pool
+    const float* uv;
+    const gpr_texture* tex;
+    const float4* texels;
+    uint32_t ntex;
+    // latitude-longitude environment (environment.h:16-28); 0 = constant env[]
+    uint32_t env_tex;
+    float env_w2o[9];
+    float env_center[3];
 };
 
 struct BvhDev {
@@ -144,13 +154,18 @@ struct gpr_scene {
     gpr_scene();
     uint32_t has_env = 0;
     float env[3] = {0, 0, 0};
+    uint32_t env_tex = 0;                                  // lat-long map texture handle
+    float env_w2o[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    float env_center[3] = {0, 0, 0};                       // world-box centroid (flatten)
+    std::vector<gpr_texture> textures;
+    std::vector<float> texels;                             // RGBA
     // flattened host copies
     std::vector<float> pos, nrm, uv;
     std::vector<uint8_t> flags;
     std::vector<uint32_t> mat, lights;
     std::vector<gpr_material> mats;
     // device copies
-    gpr::DevBuf d_pos, d_nrm, d_flags, d_mat, d_mats, d_lights, d_stats;
+    gpr::DevBuf d_pos, d_nrm, d_flags, d_mat, d_mats, d_lights, d_stats, d_uv, d_tex, d_texels;
     gpr::BvhDev bvh;
     bool flattened = false;
     bool committed = false;
@@ -174,6 +189,8 @@ int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64
                  unsigned long long* stats, hipStream_t st, int nt = 256, int sl = 16, int chunk = 64,
                  int refill = 0);
 bool trace_cfg_valid(int nt, int sl, int chunk);
+// imageio.cpp: ReadImage -- width*height*3 linear floats, bottom row first
+int read_image(const std::string& path, uint32_t* w, uint32_t* h, std::vector<float>* rgb);
 // host math shared by api.cpp and the renderer
 void camera_raster_to_camera(uint32_t w, uint32_t h, float fov_deg, float out[16]);
 void lookat_matrix(const float target[3], const float origin[3], const float up[3], float m[16]);

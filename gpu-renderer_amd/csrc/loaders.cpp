@@ -303,7 +303,7 @@ struct Props {
     std::map<std::string, F3> v;
     std::map<std::string, F3> c;
     std::map<std::string, Xform> t;
-    std::map<std::string, std::string> tex;  // texture types (unsupported)
+    std::map<std::string, uint32_t> tex;  // texture handles of nested <texture> objects
     float getf(const std::string& k, float d) const {
         auto it = f.find(k);
         if (it != f.end()) return it->second;
@@ -504,21 +504,54 @@ struct Loader {
         dst[1] = c.y;
         dst[2] = c.z;
     }
-    // createMaterial (mitsubaloader.cpp:289-441); constant textures only
+    // createTexture (mitsubaloader.cpp:530-554): bitmap (ReadImage) or 2x2 checkerboard
+    int texture(const std::string& type, const Props& L, uint32_t* id) {
+        const float off[2] = {L.getf("uoffset", 0.f), L.getf("voffset", 0.f)};
+        const float sc[2] = {L.getf("uscale", 1.f), L.getf("vscale", 1.f)};
+        if (type == "bitmap") {
+            auto it = L.s.find("filename");
+            if (it == L.s.end()) return bad(GPR_ERR_PARSE, "bitmap texture without filename");
+            const std::string f = it->second.size() && it->second[0] == '/' ? it->second : dir + it->second;
+            uint32_t w = 0, h = 0;
+            std::vector<float> px;
+            int rc = read_image(f, &w, &h, &px);
+            if (rc == GPR_OK) rc = gpr_scene_add_texture(scene, px.data(), w, h, off, sc, id);
+            return rc == GPR_OK ? GPR_OK : (err = rc);
+        } else if (type == "checkerboard") {
+            const F3 a = L.getc("color0", {0.4f, 0.4f, 0.4f}), b = L.getc("color1", {0.2f, 0.2f, 0.2f});
+            const float c0[3] = {a.x, a.y, a.z}, c1[3] = {b.x, b.y, b.z};
+            const int rc = gpr_scene_add_checkerboard(scene, c0, c1, off, sc, id);
+            return rc == GPR_OK ? GPR_OK : (err = rc);
+        }
+        return bad(GPR_ERR_UNSUPPORTED, "texture type '" + type + "'");
+    }
+    // createMaterial (mitsubaloader.cpp:289-441).  Every parameter a type sets
+    // is a texture when the property is a nested <texture>, else a constant.
     int material(const std::string& type, const Props& L) {
-        for (auto& kv : L.tex)
-            return bad(GPR_ERR_UNSUPPORTED, "texture '" + kv.second + "' for '" + kv.first + "': only constant textures are supported");
         gpr_material& m = cur_mat;
+        auto tx = [&](int slot, const char* key) {
+            auto it = L.tex.find(key);
+            m.tex[slot] = it == L.tex.end() ? 0u : it->second;
+        };
+        // Parameters a type does not read keep their previous values in the
+        // reference's current material without effect; their textures are
+        // dropped here so that the shading kernel does not look them up.
+        if (type != "disney") std::memset(m.tex, 0, sizeof(m.tex));
         if (type == "diffuse") {
             set_color(m.color, L.getc("reflectance", {0.3f, 0.3f, 0.3f}));
+            tx(GPR_TEX_COLOR, "reflectance");
             m.type = GPR_MAT_DIFFUSE;
         } else if (type == "mirror") {
             set_color(m.color, L.getc("reflectance", {1, 1, 1}));
             m.ior = L.getf("ior", 100.f);
+            tx(GPR_TEX_COLOR, "reflectance");
+            tx(GPR_TEX_IOR, "ior");
             m.type = GPR_MAT_SPECULAR;
         } else if (type == "glass") {
             set_color(m.color, L.getc("reflectance", {1, 1, 1}));
             m.ior = L.getf("ior", 1.5f);
+            tx(GPR_TEX_COLOR, "reflectance");
+            tx(GPR_TEX_IOR, "ior");
             m.type = GPR_MAT_SPECULAR;
         } else if (type == "dielectric" || type == "roughdielectric") {
             // Both map to the smooth dielectric and intIOR/extIOR are ignored: the
@@ -527,10 +560,12 @@ struct Loader {
             // through gpr_scene_add_mesh with GPR_MAT_ROUGH_DIELECTRIC.
             set_color(m.color, L.getc("reflectance", {1, 1, 1}));
             m.ior = L.getf("ior", 1.5f);
+            tx(GPR_TEX_COLOR, "reflectance");
+            tx(GPR_TEX_IOR, "ior");
             m.type = GPR_MAT_SPECULAR;
         } else if (type == "conductor" || type == "roughconductor") {
             set_color(m.color, L.getc("specularReflectance", {0.3f, 0.3f, 0.3f}));
-            const bool has_eta_i = L.c.count("etaI") != 0;
+            const bool has_eta_i = L.c.count("etaI") != 0 || L.tex.count("etaI") != 0;  // findColor: the key exists
             set_color(m.eta_i, L.getc("etaI", {1, 1, 1}));
             set_color(m.eta_t, L.getc(has_eta_i ? "etaT" : "eta", {0.200438f, 0.924033f, 1.10221f}));
             set_color(m.k, L.getc("k", {3.91295f, 2.45285f, 2.14219f}));
@@ -544,9 +579,15 @@ struct Loader {
             // both roughness textures read the "alphaU" key (mitsubaloader.cpp:368-369)
             m.alpha_x = L.hasf("alphaU") ? L.getf("alphaU", au) : au;
             m.alpha_y = L.hasf("alphaU") ? L.getf("alphaU", av) : av;
+            tx(GPR_TEX_COLOR, "specularReflectance");
+            tx(GPR_TEX_ETA_I, "etaI");
+            tx(GPR_TEX_ETA_T, has_eta_i ? "etaT" : "eta");
+            tx(GPR_TEX_K, "k");
+            tx(GPR_TEX_ALPHA_X, "alphaU");
+            tx(GPR_TEX_ALPHA_Y, "alphaU");
             m.type = GPR_MAT_ROUGH_CONDUCTOR;
         } else if (type == "roughplastic") {
-            const bool has_eta_i = L.c.count("etaI") != 0;
+            const bool has_eta_i = L.c.count("etaI") != 0 || L.tex.count("etaI") != 0;
             set_color(m.eta_i, L.getc(has_eta_i ? "etaI" : "eta", {1.5f, 1.5f, 1.5f}));
             set_color(m.eta_t, L.getc(has_eta_i ? "etaT" : "etaI", {1, 1, 1}));
             float au, av;
@@ -560,11 +601,18 @@ struct Loader {
             m.alpha_y = L.hasf("alphaU") ? L.getf("alphaU", av) : av;
             set_color(m.color, L.getc("diffuseReflectance", {0.5f, 0.5f, 0.5f}));
             set_color(m.color1, L.getc("specularReflectance", {1, 1, 1}));
+            tx(GPR_TEX_ETA_I, has_eta_i ? "etaI" : "eta");
+            tx(GPR_TEX_ETA_T, has_eta_i ? "etaT" : "etaI");
+            tx(GPR_TEX_ALPHA_X, "alphaU");
+            tx(GPR_TEX_ALPHA_Y, "alphaU");
+            tx(GPR_TEX_COLOR, "diffuseReflectance");
+            tx(GPR_TEX_COLOR1, "specularReflectance");
             m.type = GPR_MAT_ROUGH_PLASTIC;
         } else if (type == "disney") {
             // parsed but without effect in the reference (mitsubaloader.cpp:429-435)
         } else {
             set_color(m.color, {0.2f, 0.2f, 0.2f});
+            m.tex[GPR_TEX_COLOR] = 0;
             m.type = GPR_MAT_DIFFUSE;
         }
         return GPR_OK;
@@ -663,7 +711,22 @@ struct Loader {
                 const float rgb[3] = {r.x, r.y, r.z};
                 gpr_scene_set_env(scene, rgb);
             } else if (type == "envmap") {
-                return bad(GPR_ERR_UNSUPPORTED, "envmap emitter (latitude-longitude bitmap)");
+                auto it = mine.s.find("filename");
+                if (it == mine.s.end()) return bad(GPR_ERR_PARSE, "envmap emitter without filename");
+                const std::string f = it->second.size() && it->second[0] == '/' ? it->second : dir + it->second;
+                Xform o2w;
+                auto xt = mine.t.find("toWorld");
+                if (xt != mine.t.end()) o2w = xt->second;
+                uint32_t w = 0, h = 0, id = 0;
+                std::vector<float> px;
+                int rc = read_image(f, &w, &h, &px);
+                if (rc == GPR_OK) rc = gpr_scene_add_texture(scene, px.data(), w, h, nullptr, nullptr, &id);
+                if (rc == GPR_OK) rc = gpr_scene_set_env_map(scene, id, &o2w.m.m[0][0]);
+                if (rc != GPR_OK) return err = rc;
+                // Inverse(Transform) swaps m and mInv (transform.h:137-139): the
+                // worldToObject matrix is the inverse accumulated with toWorld
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) scene->env_w2o[3 * r + c] = o2w.inv.m[r][c];
             } else {
                 return bad(GPR_ERR_UNSUPPORTED, "emitter type '" + type + "'");
             }
@@ -736,7 +799,9 @@ struct Loader {
             x.inv = inverse(x.m);
             cur_xf = compose(cur_xf, x);
         } else if (tag == "texture") {
-            parent.tex[name] = type;
+            uint32_t tid = 0;
+            if (texture(type, mine, &tid) != GPR_OK) return err;
+            parent.tex[name] = tid;
         } else {
             return bad(GPR_ERR_PARSE, "unexpected tag <" + tag + ">");
         }
@@ -821,9 +886,14 @@ int load_obj_scene(const std::string& path, gpr_scene* s) {
     return gpr_scene_set_camera_lookat(s, c, o, up, 40.f, 256, 256);
 }
 
-// .gprs: little-endian binary, "GPRS" magic, version 1, then
+// .gprs: little-endian binary, "GPRS" magic, u32 version (1 or 2), then
 //   u32 nmeshes, u32 width, u32 height, f32 fov, f32 c2w[16], u32 has_env, f32 env[3]
-//   per mesh: gpr_material, u32 np, nn, nuv, ntri, f32 p[3np], n[3nn], uv[2nuv], i32 corners[9ntri]
+//   version 2 only:
+//     u32 cam_free, f32 eye[3], dir[3], up[3], fov_x, fov_y, yaw, pitch
+//     u32 ntex, per texture: u32 width, height, f32 uv_offset[2], uv_scale[2], f32 rgb[3 w h]
+//     u32 env_tex, f32 env_w2o[9]
+//   per mesh: gpr_material (version 1: without tex[]), u32 np, nn, nuv, ntri,
+//             f32 p[3np], n[3nn], uv[2nuv], i32 corners[9ntri]
 template <class T>
 bool rd(FILE* f, T* v, size_t n = 1) {
     return std::fread(v, sizeof(T), n, f) == n;
@@ -835,14 +905,34 @@ int load_gprs(const std::string& path, gpr_scene* s) {
     char magic[4];
     uint32_t ver = 0, nm = 0, w = 0, h = 0, has_env = 0;
     float fov = 0, c2w[16], env[3];
-    if (!rd(f, magic, 4) || std::memcmp(magic, "GPRS", 4) != 0 || !rd(f, &ver) || ver != 1)
-        return fail(GPR_ERR_PARSE, path + ": not a version-1 .gprs file");
+    if (!rd(f, magic, 4) || std::memcmp(magic, "GPRS", 4) != 0 || !rd(f, &ver) || (ver != 1 && ver != 2))
+        return fail(GPR_ERR_PARSE, path + ": not a version-1/2 .gprs file");
     if (!rd(f, &nm) || !rd(f, &w) || !rd(f, &h) || !rd(f, &fov) || !rd(f, c2w, 16) || !rd(f, &has_env) || !rd(f, env, 3))
         return fail(GPR_ERR_PARSE, path + ": truncated header");
+    uint32_t cam_free = 0, env_tex = 0;
+    float cam[14], w2o[9];
+    if (ver == 2) {
+        uint32_t ntex = 0;
+        if (!rd(f, &cam_free) || !rd(f, cam, 14) || !rd(f, &ntex)) return fail(GPR_ERR_PARSE, path + ": truncated header");
+        if (ntex > (1u << 20)) return fail(GPR_ERR_PARSE, path + ": implausible texture count");
+        for (uint32_t k = 0; k < ntex; ++k) {
+            uint32_t tw = 0, th = 0, id = 0;
+            float uvo[2], uvs[2];
+            if (!rd(f, &tw) || !rd(f, &th) || !rd(f, uvo, 2) || !rd(f, uvs, 2))
+                return fail(GPR_ERR_PARSE, path + ": truncated texture header");
+            if (tw == 0 || th == 0 || (uint64_t)tw * th > (1ull << 28)) return fail(GPR_ERR_PARSE, path + ": implausible texture size");
+            std::vector<float> px(3ull * tw * th);
+            if (!rd(f, px.data(), px.size())) return fail(GPR_ERR_PARSE, path + ": truncated texels");
+            GPR_TRY(gpr_scene_add_texture(s, px.data(), tw, th, uvo, uvs, &id));
+        }
+        if (!rd(f, &env_tex) || !rd(f, w2o, 9)) return fail(GPR_ERR_PARSE, path + ": truncated header");
+        if (env_tex > ntex) return fail(GPR_ERR_PARSE, path + ": environment texture out of range");
+    }
+    const size_t mat_bytes = ver == 1 ? offsetof(gpr_material, tex) : sizeof(gpr_material);
     for (uint32_t k = 0; k < nm; ++k) {
-        gpr_material m;
+        gpr_material m{};
         uint32_t np, nn, nuv, nt;
-        if (!rd(f, &m) || !rd(f, &np) || !rd(f, &nn) || !rd(f, &nuv) || !rd(f, &nt))
+        if (std::fread(&m, 1, mat_bytes, f) != mat_bytes || !rd(f, &np) || !rd(f, &nn) || !rd(f, &nuv) || !rd(f, &nt))
             return fail(GPR_ERR_PARSE, path + ": truncated mesh header");
         if (np > (1u << 30) || nn > (1u << 30) || nuv > (1u << 30) || nt > (1u << 30))
             return fail(GPR_ERR_PARSE, path + ": implausible mesh size");
@@ -855,7 +945,23 @@ int load_gprs(const std::string& path, gpr_scene* s) {
                                    c.data(), nt, &m));
     }
     if (has_env) GPR_TRY(gpr_scene_set_env(s, env));
-    return gpr_scene_set_camera(s, c2w, fov, w, h);
+    GPR_TRY(gpr_scene_set_camera(s, c2w, fov, w, h));
+    if (env_tex) {
+        s->has_env = 1;
+        s->env_tex = env_tex;
+        std::memcpy(s->env_w2o, w2o, sizeof(w2o));
+    }
+    if (cam_free) {
+        s->cam_free = true;
+        std::memcpy(s->eye, cam, 12);
+        std::memcpy(s->dir, cam + 3, 12);
+        std::memcpy(s->up, cam + 6, 12);
+        s->fovx = cam[9];
+        s->fovy = cam[10];
+        s->yaw = cam[11];
+        s->pitch = cam[12];
+    }
+    return GPR_OK;
 }
 
 }  // namespace
@@ -908,21 +1014,28 @@ extern "C" int gpr_scene_save(const gpr_scene* s, const char* path) {
     FILE* f = std::fopen(path, "wb");
     if (!f) return fail(GPR_ERR_IO, std::string("cannot open ") + path);
     std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
-    const uint32_t ver = 1, nm = (uint32_t)s->meshes.size();
-    bool ok = std::fwrite("GPRS", 1, 4, f) == 4;
-    ok = ok && std::fwrite(&ver, 4, 1, f) == 1 && std::fwrite(&nm, 4, 1, f) == 1;
-    ok = ok && std::fwrite(&s->width, 4, 1, f) == 1 && std::fwrite(&s->height, 4, 1, f) == 1;
-    ok = ok && std::fwrite(&s->fov, 4, 1, f) == 1 && std::fwrite(s->c2w, 4, 16, f) == 16;
-    ok = ok && std::fwrite(&s->has_env, 4, 1, f) == 1 && std::fwrite(s->env, 4, 3, f) == 3;
+    const uint32_t ver = 2, nm = (uint32_t)s->meshes.size();
+    auto put = [&](const void* p, size_t bytes) { return bytes == 0 || std::fwrite(p, 1, bytes, f) == bytes; };
+    bool ok = put("GPRS", 4) && put(&ver, 4) && put(&nm, 4) && put(&s->width, 4) && put(&s->height, 4) &&
+              put(&s->fov, 4) && put(s->c2w, 64) && put(&s->has_env, 4) && put(s->env, 12);
+    const uint32_t cam_free = s->cam_free ? 1u : 0u, ntex = (uint32_t)s->textures.size();
+    const float cam[14] = {s->eye[0], s->eye[1], s->eye[2], s->dir[0], s->dir[1], s->dir[2], s->up[0], s->up[1], s->up[2],
+                           s->fovx, s->fovy, s->yaw, s->pitch, 0.f};
+    ok = ok && put(&cam_free, 4) && put(cam, sizeof(cam)) && put(&ntex, 4);
+    for (const gpr_texture& t : s->textures) {
+        ok = ok && put(&t.width, 4) && put(&t.height, 4) && put(t.uv_offset, 8) && put(t.uv_scale, 8);
+        std::vector<float> px(3ull * t.width * t.height);
+        for (size_t i = 0; i < (size_t)t.width * t.height; ++i)
+            for (int c = 0; c < 3; ++c) px[3 * i + c] = s->texels[4 * (t.offset + i) + c];
+        ok = ok && put(px.data(), 4 * px.size());
+    }
+    const uint32_t env_tex = s->has_env ? s->env_tex : 0u;
+    ok = ok && put(&env_tex, 4) && put(s->env_w2o, 36);
     for (const Mesh& m : s->meshes) {
         const uint32_t np = (uint32_t)(m.p.size() / 3), nn = (uint32_t)(m.n.size() / 3), nuv = (uint32_t)(m.uv.size() / 2);
-        ok = ok && std::fwrite(&m.mat, sizeof(gpr_material), 1, f) == 1;
-        ok = ok && std::fwrite(&np, 4, 1, f) == 1 && std::fwrite(&nn, 4, 1, f) == 1 && std::fwrite(&nuv, 4, 1, f) == 1 &&
-             std::fwrite(&m.ntri, 4, 1, f) == 1;
-        ok = ok && std::fwrite(m.p.data(), 4, m.p.size(), f) == m.p.size();
-        ok = ok && std::fwrite(m.n.data(), 4, m.n.size(), f) == m.n.size();
-        ok = ok && std::fwrite(m.uv.data(), 4, m.uv.size(), f) == m.uv.size();
-        ok = ok && std::fwrite(m.corners.data(), 4, m.corners.size(), f) == m.corners.size();
+        ok = ok && put(&m.mat, sizeof(gpr_material)) && put(&np, 4) && put(&nn, 4) && put(&nuv, 4) && put(&m.ntri, 4);
+        ok = ok && put(m.p.data(), 4 * m.p.size()) && put(m.n.data(), 4 * m.n.size()) && put(m.uv.data(), 4 * m.uv.size()) &&
+             put(m.corners.data(), 4 * m.corners.size());
     }
     if (!ok) return fail(GPR_ERR_IO, std::string("short write to ") + path);
     return GPR_OK;

@@ -31,6 +31,7 @@
 #include <memory>
 
 #include "dev_bsdf.h"
+#include "dev_texture.h"
 #include "engine.h"
 #include "traverse.h"
 
@@ -535,7 +536,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
             const uint32_t t = (uint32_t)hid;
             const Geom g = setup_geom(sv, t, hh.z, hh.w);
             const v3 wo = -norm3(rd);
-            const gpr_material m = sv.mats[sv.mat[t]];
+            gpr_material m = sv.mats[sv.mat[t]];
+            if (has_textures(m)) resolve_textures(sv, m, t, hh.z, hh.w);
             const Frame fr = make_frame(g.ns);
             if (bounce == 0 || spec) {
                 const c3 le = loadc(m.emission);
@@ -599,7 +601,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
                 }
             }
         } else if (sv.has_env) {
-            fb_add(a.fb, pix, w * C(sv.env[0], sv.env[1], sv.env[2]));
+            c3 le = C(sv.env[0], sv.env[1], sv.env[2]);
+            if (sv.env_tex) {
+                const float4 ot = qi.o_tmin[i];
+                le = env_le(sv, V(ot.x, ot.y, ot.z), rd);
+            }
+            fb_add(a.fb, pix, w * le);
         }
     }
     // ---- order-preserving compaction of both appends
@@ -1027,14 +1034,10 @@ int launch_sample(gpr_renderer* r, hipStream_t st) {
 
 // The sample sequence captured once into a hipGraph and replayed per sample;
 // captured again when the scene view (camera included) has changed.
-bool same_view(const SceneView& a, const SceneView& b) {
-    return a.ntri == b.ntri && a.nlights == b.nlights && a.pos == b.pos && a.nrm == b.nrm && a.flags == b.flags &&
-           a.mat == b.mat && a.mats == b.mats && a.lights == b.lights && a.ref == b.ref && a.bvh4 == b.bvh4 &&
-           a.tris == b.tris && a.root4 == b.root4 && a.nbvh4 == b.nbvh4 && a.nref == b.nref &&
-           a.has_env == b.has_env && std::memcmp(a.env, b.env, sizeof(a.env)) == 0 &&
-           std::memcmp(a.r2c, b.r2c, sizeof(a.r2c)) == 0 && std::memcmp(a.c2w, b.c2w, sizeof(a.c2w)) == 0 &&
-           a.width == b.width && a.height == b.height;
-}
+// Every field of the view is baked into the captured graph's kernel
+// arguments; gpr_scene::view() value-initialises the struct, so padding is
+// zero and a byte comparison covers all of them.
+bool same_view(const SceneView& a, const SceneView& b) { return std::memcmp(&a, &b, sizeof(SceneView)) == 0; }
 
 int sample_graph(gpr_renderer* r, hipGraphExec_t* out) {
     const SceneView now = r->scene->view();

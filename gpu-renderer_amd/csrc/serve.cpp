@@ -37,11 +37,18 @@ struct Arena {
     std::vector<SceneView> views;
 };
 
+// uvs, texture table and texels travel only with scenes that have textures
+size_t texture_bytes(const gpr_scene* s) {
+    if (s->textures.empty()) return 0;
+    return align256(24 * s->mat.size()) + align256(sizeof(gpr_texture) * s->textures.size()) +
+           align256(4 * s->texels.size());
+}
+
 size_t scene_bytes(const gpr_scene* s) {
     const size_t n = s->mat.size();
     return align256(36 * n) * 2 + align256(n) + align256(4 * n) + align256(sizeof(gpr_material) * s->mats.size()) +
            align256(4 * std::max<size_t>(1, s->lights.size())) + align256(sizeof(Bvh4Node) * s->bvh.nbvh4) +
-           align256(sizeof(TriRec) * n) + align256(sizeof(gpr_ref_node) * (n ? 2 * n - 1 : 0));
+           align256(sizeof(TriRec) * n) + align256(sizeof(gpr_ref_node) * (n ? 2 * n - 1 : 0)) + texture_bytes(s);
 }
 
 int arena_add(Arena& A, const gpr_scene* s, hipStream_t st, uint32_t* id) {
@@ -74,6 +81,12 @@ int arena_add(Arena& A, const gpr_scene* s, hipStream_t st, uint32_t* id) {
     a.ref = reinterpret_cast<const gpr_ref_node*>(put(v.ref, sizeof(gpr_ref_node) * (2 * n - 1)));
     if (!a.pos || !a.nrm || !a.flags || !a.mat || !a.mats || !a.lights || !a.bvh4 || !a.tris || !a.ref)
         return fail(GPR_ERR_HIP, "add_scene: arena copy failed");
+    if (!s->textures.empty()) {
+        a.uv = reinterpret_cast<const float*>(put(v.uv, 24 * n));
+        a.tex = reinterpret_cast<const gpr_texture*>(put(v.tex, sizeof(gpr_texture) * s->textures.size()));
+        a.texels = reinterpret_cast<const float4*>(put(v.texels, 4 * s->texels.size()));
+        if (!a.uv || !a.tex || !a.texels) return fail(GPR_ERR_HIP, "add_scene: arena copy failed");
+    }
     if (off > need) return fail(GPR_ERR_INVALID, "add_scene: arena layout mismatch");
     A.used += need;
     *id = (uint32_t)A.views.size();

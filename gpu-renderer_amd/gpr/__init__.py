@@ -30,14 +30,18 @@ class Material(C.Structure):
     _fields_ = [("type", C.c_uint32), ("color", C.c_float * 3), ("color1", C.c_float * 3),
                 ("emission", C.c_float * 3), ("ior", C.c_float), ("alpha_x", C.c_float),
                 ("alpha_y", C.c_float), ("eta_i", C.c_float * 3), ("eta_t", C.c_float * 3),
-                ("k", C.c_float * 3)]
+                ("k", C.c_float * 3), ("tex", C.c_uint32 * 8)]
 
 
 MATERIAL_DTYPE = np.dtype([("type", "<u4"), ("color", "<f4", 3), ("color1", "<f4", 3),
                            ("emission", "<f4", 3), ("ior", "<f4"), ("alpha_x", "<f4"),
                            ("alpha_y", "<f4"), ("eta_i", "<f4", 3), ("eta_t", "<f4", 3),
-                           ("k", "<f4", 3)])
+                           ("k", "<f4", 3), ("tex", "<u4", 8)])
 assert MATERIAL_DTYPE.itemsize == C.sizeof(Material)
+# texture slots of Material.tex (a handle from Scene.add_texture; 0 = the constant field)
+TEX_COLOR, TEX_COLOR1, TEX_IOR, TEX_ALPHA_X, TEX_ALPHA_Y, TEX_ETA_I, TEX_ETA_T, TEX_K = range(8)
+TEXTURE_DTYPE = np.dtype([("offset", "<u4"), ("width", "<u4"), ("height", "<u4"), ("pad", "<u4"),
+                          ("uv_offset", "<f4", 2), ("uv_scale", "<f4", 2)])
 
 
 class SceneInfo(C.Structure):
@@ -47,7 +51,9 @@ class SceneInfo(C.Structure):
                 ("bvh4_nodes", C.c_uint32), ("committed", C.c_uint32),
                 ("cam_free", C.c_uint32), ("eye", C.c_float * 3), ("dir", C.c_float * 3),
                 ("up", C.c_float * 3), ("fov_x", C.c_float), ("fov_y", C.c_float),
-                ("yaw", C.c_float), ("pitch", C.c_float)]
+                ("yaw", C.c_float), ("pitch", C.c_float),
+                ("ntex", C.c_uint32), ("ntexel", C.c_uint32), ("env_tex", C.c_uint32),
+                ("env_w2o", C.c_float * 9), ("env_center", C.c_float * 3)]
 
 
 class RenderOpts(C.Structure):
@@ -127,6 +133,13 @@ _SIG = {
     "gpr_renderer_resize": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
     "gpr_renderer_get_resolution": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "gpr_scene_set_env": (C.c_int, [_P, _P]),
+    "gpr_scene_add_texture": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, C.POINTER(C.c_uint32)]),
+    "gpr_scene_add_checkerboard": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(C.c_uint32)]),
+    "gpr_scene_set_env_map": (C.c_int, [_P, C.c_uint32, _P]),
+    "gpr_scene_export_textures": (C.c_int, [_P, _P, _P]),
+    "gpr_image_read": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32)]),
+    "gpr_image_free": (None, [C.POINTER(C.c_float)]),
     "gpr_scene_commit": (C.c_int, [_P]),
     "gpr_scene_free": (C.c_int, [_P]),
     "gpr_scene_get_info": (C.c_int, [_P, C.POINTER(SceneInfo)]),
@@ -197,7 +210,8 @@ def _ptr(a):
 
 def material(type=MAT_DIFFUSE, color=(0.5, 0.5, 0.5), color1=(1, 1, 1), emission=(0, 0, 0),
              ior=1.5, alpha=0.1, alpha_y=None, eta_i=(1, 1, 1), eta_t=(0.200438, 0.924033, 1.10221),
-             k=(3.91295, 2.45285, 2.14219)):
+             k=(3.91295, 2.45285, 2.14219), tex=None):
+    """tex: {slot: texture handle} with slots TEX_COLOR ... TEX_K."""
     m = Material()
     m.type = type
     m.color[:] = color
@@ -209,7 +223,22 @@ def material(type=MAT_DIFFUSE, color=(0.5, 0.5, 0.5), color1=(1, 1, 1), emission
     m.eta_i[:] = eta_i
     m.eta_t[:] = eta_t
     m.k[:] = k
+    for slot, h in (tex or {}).items():
+        m.tex[slot] = h
     return m
+
+
+def read_image(path):
+    """ReadImage: (height, width, 3) linear float32, row 0 = bottom row of the file."""
+    p = C.POINTER(C.c_float)()
+    w, h = C.c_uint32(), C.c_uint32()
+    check(lib.gpr_image_read(os.fsencode(path), C.byref(p), C.byref(w), C.byref(h)))
+    try:
+        n = w.value * h.value * 3
+        out = np.ctypeslib.as_array(p, shape=(n,)).copy().reshape(h.value, w.value, 3)
+    finally:
+        lib.gpr_image_free(p)
+    return out
 
 
 # ------------------------------------------------------------ generators --
@@ -321,6 +350,37 @@ class Scene:
         a = np.asarray(rgb, np.float32)
         check(lib.gpr_scene_set_env(self.h, _ptr(a)))
 
+    def add_texture(self, rgb, uv_offset=None, uv_scale=None):
+        """rgb: (height, width, 3) linear texels, row 0 at v = 0.  Returns the handle."""
+        a = np.ascontiguousarray(rgb, np.float32)
+        if a.ndim != 3 or a.shape[2] != 3:
+            raise ValueError("texture must be (height, width, 3)")
+        o = None if uv_offset is None else np.ascontiguousarray(uv_offset, np.float32)
+        sc = None if uv_scale is None else np.ascontiguousarray(uv_scale, np.float32)
+        out = C.c_uint32()
+        check(lib.gpr_scene_add_texture(self.h, _ptr(a), a.shape[1], a.shape[0], _ptr(o), _ptr(sc), C.byref(out)))
+        return out.value
+
+    def add_checkerboard(self, c0=(0.4, 0.4, 0.4), c1=(0.2, 0.2, 0.2), uv_offset=None, uv_scale=None):
+        a, b = (np.ascontiguousarray(c, np.float32) for c in (c0, c1))
+        o = None if uv_offset is None else np.ascontiguousarray(uv_offset, np.float32)
+        sc = None if uv_scale is None else np.ascontiguousarray(uv_scale, np.float32)
+        out = C.c_uint32()
+        check(lib.gpr_scene_add_checkerboard(self.h, _ptr(a), _ptr(b), _ptr(o), _ptr(sc), C.byref(out)))
+        return out.value
+
+    def set_env_map(self, tex, o2w=None):
+        m = None if o2w is None else np.ascontiguousarray(np.asarray(o2w, np.float32).reshape(16))
+        check(lib.gpr_scene_set_env_map(self.h, tex, _ptr(m)))
+
+    def textures(self):
+        """(records TEXTURE_DTYPE, texels (ntexel, 4) float32)"""
+        i = self.info()
+        recs = np.zeros(max(i.ntex, 1), TEXTURE_DTYPE)
+        texels = np.zeros((max(i.ntexel, 1), 4), np.float32)
+        check(lib.gpr_scene_export_textures(self.h, _ptr(recs), _ptr(texels)))
+        return recs[:i.ntex], texels[:i.ntexel]
+
     def commit(self):
         check(lib.gpr_scene_commit(self.h))
         return self
@@ -341,7 +401,9 @@ class Scene:
         mats = np.empty(max(i.nmat, 1), MATERIAL_DTYPE)
         check(lib.gpr_scene_export_flat(self.h, _ptr(pos), _ptr(nrm), _ptr(uv), _ptr(flags),
                                         _ptr(mat), _ptr(mats)))
-        return dict(pos=pos, nrm=nrm, uv=uv, flags=flags, mat=mat, mats=mats[:i.nmat], info=i)
+        recs, texels = self.textures()
+        return dict(pos=pos, nrm=nrm, uv=uv, flags=flags, mat=mat, mats=mats[:i.nmat], info=i,
+                    tex=recs, texels=texels)
 
     def build(self):
         check(lib.gpr_bvh_build(self.h))

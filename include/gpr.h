@@ -59,8 +59,12 @@ typedef struct {
     uint32_t leaf;
 } gpr_ref_node;
 
-/* Material with constant textures -- Material (src/renderer/material.h:6-35),
- * factories in src/renderer/loaders/mitsubaloader.cpp:289-441. */
+/* Material -- Material (src/renderer/material.h:6-35), factories in
+ * src/renderer/loaders/mitsubaloader.cpp:289-441.  Every parameter is a
+ * TextureView in the reference; here the constant fields hold the value and
+ * tex[slot] (a handle from gpr_scene_add_texture, 0 = constant) replaces a
+ * field by a nearest-texel lookup at the hit's uv.  Float parameters (ior,
+ * alpha_x, alpha_y) read the red channel (TextureView::evalFloat). */
 #define GPR_MAT_DIFFUSE 0          /* Lambert */
 #define GPR_MAT_SPECULAR 1         /* smooth dielectric / mirror (FresnelSpecular) */
 #define GPR_MAT_ROUGH_CONDUCTOR 2  /* GGX reflection, conductor Fresnel */
@@ -74,7 +78,28 @@ typedef struct {
     float ior;
     float alpha_x, alpha_y;
     float eta_i[3], eta_t[3], k[3];
+    uint32_t tex[8];   /* GPR_TEX_* slot -> texture handle (0 = constant field) */
 } gpr_material;
+#define GPR_TEX_COLOR 0
+#define GPR_TEX_COLOR1 1
+#define GPR_TEX_IOR 2
+#define GPR_TEX_ALPHA_X 3
+#define GPR_TEX_ALPHA_Y 4
+#define GPR_TEX_ETA_I 5
+#define GPR_TEX_ETA_T 6
+#define GPR_TEX_K 7
+#define GPR_TEX_SLOTS 8
+
+/* Texture record (TextureView, src/renderer/texture.h:14-35): texels
+ * [offset, offset + width*height) of the scene's RGBA float pool, row-major,
+ * row 0 at v = 0 (the reference's images are flipped vertically on load). */
+typedef struct {
+    uint32_t offset;
+    uint32_t width, height;
+    uint32_t pad;
+    float uv_offset[2];
+    float uv_scale[2];
+} gpr_texture;
 
 typedef struct gpr_scene gpr_scene;
 typedef struct gpr_renderer gpr_renderer;
@@ -114,6 +139,30 @@ int gpr_scene_set_camera_lookat(gpr_scene* s, const float target[3], const float
 /* Constant environment light (ParseRecord::createLight "constant",
  * mitsubaloader.cpp:467-471). */
 int gpr_scene_set_env(gpr_scene* s, const float rgb[3]);
+/* Texture (src/renderer/texture.cpp:16-62): width*height RGB texels,
+ * row-major, row 0 at v = 0.  uv_offset / uv_scale may be NULL (0 / 1).
+ * *out_id receives the handle (>= 1) for gpr_material.tex[].  Textures must be
+ * added before the meshes that use them. */
+int gpr_scene_add_texture(gpr_scene* s, const float* rgb, uint32_t width, uint32_t height,
+                          const float uv_offset[2], const float uv_scale[2], uint32_t* out_id);
+/* Checkerboard texture: 2x2 texels c0 c1 / c1 c0 (texture.cpp:50-62). */
+int gpr_scene_add_checkerboard(gpr_scene* s, const float c0[3], const float c1[3], const float uv_offset[2],
+                               const float uv_scale[2], uint32_t* out_id);
+/* Latitude-longitude environment light (EnvironmentLight, environment.h:10-53,
+ * ParseRecord::createLight "envmap", mitsubaloader.cpp:472-478): texture
+ * tex_id, o2w the row-major toWorld matrix (NULL = identity).  Replaces a
+ * constant environment light and vice versa. */
+int gpr_scene_set_env_map(gpr_scene* s, uint32_t tex_id, const float o2w[16]);
+/* Host copies of the texture table (ntex records) and the texel pool
+ * (ntexel RGBA floats), sizes in gpr_scene_info.  Either pointer may be NULL. */
+int gpr_scene_export_textures(const gpr_scene* s, gpr_texture* recs, float* texels_rgba);
+/* ReadImage (src/renderer/imageio.cpp:45-57, stbi_loadf with a vertical
+ * flip): PNG (8/16-bit, non-interlaced), Radiance .hdr and binary PNM (P5/P6)
+ * -> *rgb, width*height*3 linear floats, row 0 = bottom row of the file.  8-bit
+ * data is linearised as (c/255)^2.2, 16-bit PNG samples keep their high byte
+ * first.  Free with gpr_image_free. */
+int gpr_image_read(const char* path, float** rgb, uint32_t* width, uint32_t* height);
+void gpr_image_free(float* rgb);
 /* Flatten meshes, upload, build the light list (scene.cpp:16-20) and the
  * BVH on the device (replaces LBVH_build, src/renderer/bvh.h:186-221). */
 int gpr_scene_commit(gpr_scene* s);
@@ -137,6 +186,11 @@ typedef struct {
     float eye[3], dir[3], up[3];
     float fov_x, fov_y;  /* radians (film axes of the free camera) */
     float yaw, pitch;    /* free-camera orientation state (Camera::rotate) */
+    uint32_t ntex;       /* texture records */
+    uint32_t ntexel;     /* RGBA texels in the pool */
+    uint32_t env_tex;    /* latitude-longitude environment texture (0 = none) */
+    float env_w2o[9];    /* its worldToObject rotation part (row-major 3x3) */
+    float env_center[3]; /* world-box centroid (scene.cpp:26-27), set by flatten */
 } gpr_scene_info;
 int gpr_scene_get_info(const gpr_scene* s, gpr_scene_info* out);
 /* Flattened per-triangle data in triangle-id order (what the kernels read):

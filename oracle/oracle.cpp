@@ -1022,6 +1022,74 @@ inline float tri_area(const orc_scene* sc, uint32_t tri) {
     return length(cross(p1 - p0, p2 - p0)) * 0.5f;
 }
 inline rgb emission(const orc_scene* sc, uint32_t tri) { return ldc(sc->mats[sc->mat[tri]].emission); }
+
+// float -> int of the GPU conversion instruction the reference's kernels run
+// on: truncate, saturate, NaN -> 0 (a C++ cast leaves these undefined)
+inline int cvt_int(float x) {
+    if (std::isnan(x)) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int)x;
+}
+// TextureView::evalSpectrum (texture.h:37-44): uv' = uv * scale + offset,
+// u2 = int(uv'.x * width) % width where width is uint32, so the int operand is
+// converted to uint32 before the remainder; idx = v2 * width + u2.
+inline rgb tex_eval(const orc_scene* sc, uint32_t handle, float u, float v) {
+    const orc_texture& t = sc->tex[handle - 1];
+    float nu = u * t.uv_scale[0] + t.uv_offset[0];
+    float nv = v * t.uv_scale[1] + t.uv_offset[1];
+    uint32_t u2 = (uint32_t)cvt_int(nu * (float)t.width) % t.width;
+    uint32_t v2 = (uint32_t)cvt_int(nv * (float)t.height) % t.height;
+    const float* px = sc->texels + 4ull * (t.offset + v2 * t.width + u2);
+    return rgb{px[0], px[1], px[2]};
+}
+// The BxDF factories (bsdfs/bxdf.h:160-218) evaluate every material
+// parameter at the hit's uv: Triangle::setupDifferential (triangle.h:126-134)
+// interpolates the corner uvs when the mesh has them, else uses the barycentrics.
+inline orc_material textured(const orc_scene* sc, const orc_material& m0, uint32_t tri, float bu, float bv) {
+    orc_material m = m0;
+    bool any = false;
+    for (int k = 0; k < 8; ++k) any = any || m.tex[k] != 0;
+    if (!any) return m;
+    float u = bu, v = bv;
+    if (sc->flags && (sc->flags[tri] & 2)) {
+        const float* q = sc->uv + 6ull * tri;
+        u = q[0] * (1 - bu - bv) + q[2] * bu + q[4] * bv;
+        v = q[1] * (1 - bu - bv) + q[3] * bu + q[5] * bv;
+    }
+    float* spectra[8] = {m.color, m.color1, nullptr, nullptr, nullptr, m.eta_i, m.eta_t, m.k};
+    float* scalars[8] = {nullptr, nullptr, &m.ior, &m.alpha_x, &m.alpha_y, nullptr, nullptr, nullptr};
+    for (int k = 0; k < 8; ++k) {
+        if (!m.tex[k]) continue;
+        rgb c = tex_eval(sc, m.tex[k], u, v);
+        if (spectra[k]) {
+            spectra[k][0] = c.r; spectra[k][1] = c.g; spectra[k][2] = c.b;
+        } else {
+            *scalars[k] = c.r;  // evalFloat reads the red channel (texture.h:46-52)
+        }
+    }
+    return m;
+}
+// EnvironmentLightView::Le (environment.h:16-28); spherical_theta/phi of
+// helper_math (acos of the clamped z, atan2 shifted into [0, 2 pi)).  acos and
+// atan2 are taken in double and rounded once.
+inline rgb env_le(const orc_scene* sc, f3 o, f3 d) {
+    f3 w = normalize(o + d * 100000.f - ld3(sc->env_center));
+    const float* m = sc->env_w2o;
+    w = normalize(f3{m[0] * w.x + m[1] * w.y + m[2] * w.z, m[3] * w.x + m[4] * w.y + m[5] * w.z,
+                     m[6] * w.x + m[7] * w.y + m[8] * w.z});
+    float z = std::fmax(-1.f, std::fmin(w.z, 1.f));
+    float theta = (float)std::acos((double)z);
+    float phi = (float)std::atan2((double)w.y, (double)w.x);
+    if (phi < 0) phi = phi + 2 * 3.14159265358979323846f;
+    float u = phi * 0.15915494309189533577f;  // INV_2_PI (fwd.h:25)
+    float v = theta * 0.31830988618379067154f;  // INV_PI (fwd.h:24)
+    const orc_texture& t = sc->tex[sc->env_tex - 1];
+    int W = (int)t.width, H = (int)t.height;
+    int u2 = std::min(W - 1, cvt_int(u * (float)W)), v2 = std::min(H - 1, cvt_int(v * (float)H));
+    const float* px = sc->texels + 4ull * (t.offset + (uint32_t)(v2 * W + u2));
+    return rgb{px[0], px[1], px[2]};
+}
 inline bool is_light(const orc_scene* sc, uint32_t tri) { return !is_black(emission(sc, tri)); }
 
 // PowerHeuristic (pathtracer_kernel.h:480-486)
@@ -1072,7 +1140,7 @@ void shade(const Ctx& c, const QE& e, uint32_t bounce, const float* samples, flo
         uint32_t tri = (uint32_t)e.hit.id;
         Geom g = setup_geom(sc, tri, e.hit.u, e.hit.v);
         f3 wo = -normalize(e.d);
-        const orc_material& mat = sc->mats[sc->mat[tri]];
+        const orc_material mat = textured(sc, sc->mats[sc->mat[tri]], tri, e.hit.u, e.hit.v);
         BSDF b = make_bsdf(mat, g.ns);
         if (bounce == 0 || e.spec) {
             if (is_light(sc, tri)) {
@@ -1145,7 +1213,8 @@ void shade(const Ctx& c, const QE& e, uint32_t bounce, const float* samples, flo
             }
         }
     } else if (sc->has_env) {
-        rgb add = e.w * ldc(sc->env);
+        rgb le = sc->env_tex ? env_le(sc, e.o, e.d) : ldc(sc->env);
+        rgb add = e.w * le;
         px[0] = px[0] + add.r; px[1] = px[1] + add.g; px[2] = px[2] + add.b;
     }
 }
@@ -1475,6 +1544,16 @@ void orc_camera_ray(const orc_scene* s, float px, float py, orc_ray* ray) {
     ray->tmax = 1e34f;
 }
 
+void orc_tex_eval(const orc_scene* s, uint32_t handle, float u, float v, float out[3]) {
+    rgb c = tex_eval(s, handle, u, v);
+    out[0] = c.r; out[1] = c.g; out[2] = c.b;
+}
+
+void orc_env_le(const orc_scene* s, const float o[3], const float d[3], float out[3]) {
+    rgb c = env_le(s, ld3(o), ld3(d));
+    out[0] = c.r; out[1] = c.g; out[2] = c.b;
+}
+
 void orc_bsdf_eval(const orc_material* m, const float n[3], const float wo[3], const float wi[3],
                    float f[3], float* pdf) {
     BSDF b = make_bsdf(*m, ld3(n));
@@ -1516,7 +1595,8 @@ int orc_render(const orc_scene* scene, const orc_node* nodes, const orc_render_o
     c.cam = make_camera(scene);
     uint32_t npix = scene->width * scene->height;
     if (scene->ntri == 0) {
-        // nothing to hit: only the environment term
+        // nothing to hit: only the environment term (a constant one)
+        if (scene->env_tex) return -1;
         for (uint32_t s = 0; s < opts->spp; ++s)
             for (uint32_t p = 0; p < npix; ++p) {
                 if (scene->has_env)

@@ -33,7 +33,9 @@ typedef struct {
 typedef struct { float o[3]; float d[3]; float tmin; float tmax; } orc_ray;
 typedef struct { float t; int32_t id; float u; float v; } orc_hit;
 
-/* Material (src/renderer/material.h:6-35) with constant textures. */
+/* Material (src/renderer/material.h:6-35): constant fields, and per
+ * parameter a texture handle (1-based, 0 = the constant) in the slot order
+ * color, color1, ior, alpha_x, alpha_y, eta_i, eta_t, k. */
 typedef struct {
     uint32_t type;        /* 0 diffuse, 1 specular, 2 rough conductor, 3 rough dielectric, 4 rough plastic */
     float color[3];
@@ -42,7 +44,15 @@ typedef struct {
     float ior;
     float alpha_x, alpha_y;
     float eta_i[3], eta_t[3], k[3];
+    uint32_t tex[8];
 } orc_material;
+
+/* TextureView (src/renderer/texture.h:14-35): texels [offset, offset+w*h) of
+ * an RGBA float pool, row-major, row 0 at v = 0. */
+typedef struct {
+    uint32_t offset, width, height, pad;
+    float uv_offset[2], uv_scale[2];
+} orc_texture;
 
 /* Flat triangle scene: the global triangle order of Scene::Scene
  * (src/renderer/scene.cpp:4-14): mesh load order, then face order. */
@@ -66,6 +76,13 @@ typedef struct {
     uint32_t cam_free;
     float eye[3], dir[3], up[3];
     float fov_x, fov_y;
+    /* textures and the latitude-longitude environment (environment.h:10-35) */
+    const orc_texture* tex;
+    const float* texels;    /* RGBA */
+    uint32_t ntex;
+    uint32_t env_tex;       /* 1-based texture handle of the lat-long map, 0 = constant env[] */
+    float env_w2o[9];       /* worldToObject, upper-left 3x3, row-major */
+    float env_center[3];    /* world-box centroid (scene.cpp:26-27) */
 } orc_scene;
 
 typedef struct {
@@ -119,6 +136,10 @@ void orc_camera_ray(const orc_scene* s, float px, float py, orc_ray* ray);
 /* --- BSDF (bsdf.h and the bsdfs directory); frame-free entry points for tests --- */
 /* eval f*|cos|, pdf, sample in the shading frame of normal n; u2 is the lobe-choice
  * uniform of the rough dielectric (samples[4] in the integrator) */
+/* TextureView::evalSpectrum (texture.h:37-44) of texture `handle` at uv. */
+void orc_tex_eval(const orc_scene* s, uint32_t handle, float u, float v, float out[3]);
+/* EnvironmentLightView::Le (environment.h:16-28) for a ray o + t d. */
+void orc_env_le(const orc_scene* s, const float o[3], const float d[3], float out[3]);
 void orc_bsdf_eval(const orc_material* m, const float n[3], const float wo[3],
                    const float wi[3], float f[3], float* pdf);
 int  orc_bsdf_sample(const orc_material* m, const float n[3], const float wo[3],

@@ -13,7 +13,7 @@ HIT_DTYPE = np.dtype([("t", "<f4"), ("id", "<i4"), ("u", "<f4"), ("v", "<f4")])
 MATERIAL_DTYPE = np.dtype([("type", "<u4"), ("color", "<f4", 3), ("color1", "<f4", 3),
                            ("emission", "<f4", 3), ("ior", "<f4"), ("alpha_x", "<f4"),
                            ("alpha_y", "<f4"), ("eta_i", "<f4", 3), ("eta_t", "<f4", 3),
-                           ("k", "<f4", 3)])
+                           ("k", "<f4", 3), ("tex", "<u4", 8)])
 
 
 def _build():
@@ -33,7 +33,9 @@ class Scene(C.Structure):
                 ("fov_deg", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32),
                 ("has_env", C.c_uint32), ("env", C.c_float * 3),
                 ("cam_free", C.c_uint32), ("eye", C.c_float * 3), ("dir", C.c_float * 3), ("up", C.c_float * 3),
-                ("fov_x", C.c_float), ("fov_y", C.c_float)]
+                ("fov_x", C.c_float), ("fov_y", C.c_float),
+                ("tex", P), ("texels", P), ("ntex", C.c_uint32), ("env_tex", C.c_uint32),
+                ("env_w2o", C.c_float * 9), ("env_center", C.c_float * 3)]
 
 
 class RenderOpts(C.Structure):
@@ -84,6 +86,10 @@ lib.orc_ggx_d.restype = C.c_float
 lib.orc_ggx_d.argtypes = [P, C.c_float, C.c_float]
 lib.orc_ggx_g.restype = C.c_float
 lib.orc_ggx_g.argtypes = [P, P, C.c_float, C.c_float]
+lib.orc_tex_eval.restype = None
+lib.orc_tex_eval.argtypes = [C.POINTER(Scene), C.c_uint32, C.c_float, C.c_float, P]
+lib.orc_env_le.restype = None
+lib.orc_env_le.argtypes = [C.POINTER(Scene), P, P, P]
 lib.orc_render.restype = C.c_int
 lib.orc_render.argtypes = [C.POINTER(Scene), P, C.POINTER(RenderOpts), P, P, C.c_int]
 
@@ -165,6 +171,14 @@ def make_scene(flat, c2w, fov, w, h, env=None, info=None):
     if env is not None:
         s.has_env = 1
         s.env[:] = env
+    if "tex" in flat and len(flat["tex"]):
+        keep["tex"] = np.ascontiguousarray(flat["tex"])
+        keep["texels"] = np.ascontiguousarray(flat["texels"], np.float32)
+        s.tex, s.texels, s.ntex = ptr(keep["tex"]), ptr(keep["texels"]), len(keep["tex"])
+    if info is not None:
+        s.env_tex = info.env_tex
+        s.env_w2o[:] = list(info.env_w2o)
+        s.env_center[:] = list(info.env_center)
     if info is not None and info.cam_free:
         s.cam_free = 1
         s.eye[:] = list(info.eye)

@@ -288,9 +288,14 @@ def test_emitter_applies_to_next_shape_only(tmp_path):
     ("<shape type=\"rectangle\"><ref id=\"nope\"/></shape>", gpr.ERR_PARSE, "nope"),
     ("<shape type=\"rectangle\"></shap>", gpr.ERR_PARSE, "mismatched"),
     ("<shape type=\"rectangle\"><float name=\"r\" value=\"1x\"/></shape>", gpr.ERR_PARSE, "float"),
-    ("<bsdf type=\"diffuse\"><texture name=\"reflectance\" type=\"bitmap\"/></bsdf>", gpr.ERR_UNSUPPORTED, "texture"),
+    ("<bsdf type=\"diffuse\"><texture name=\"reflectance\" type=\"bitmap\"/></bsdf>", gpr.ERR_PARSE, "filename"),
+    ("<bsdf type=\"diffuse\"><texture name=\"reflectance\" type=\"wood\"/></bsdf>", gpr.ERR_UNSUPPORTED, "wood"),
+    ("<bsdf type=\"diffuse\"><texture name=\"reflectance\" type=\"bitmap\">"
+     "<string name=\"filename\" value=\"missing.png\"/></texture></bsdf>", gpr.ERR_IO, "missing.png"),
     ("<shape type=\"obj\"><string name=\"filename\" value=\"missing.obj\"/></shape>", gpr.ERR_IO, "missing.obj"),
-    ("<emitter type=\"envmap\"/>", gpr.ERR_UNSUPPORTED, "envmap"),
+    ("<emitter type=\"envmap\"/>", gpr.ERR_PARSE, "filename"),
+    ("<emitter type=\"envmap\"><string name=\"filename\" value=\"bad.xml\"/></emitter>", gpr.ERR_UNSUPPORTED,
+     "not recognised"),
 ])
 def test_xml_errors(tmp_path, body, code, needle):
     p = write(tmp_path, "bad.xml", SCENE_HEAD + body + "\n</scene>")
