@@ -40,6 +40,7 @@ constexpr int kLocalThreads = 256;
 constexpr int kFastCap = 1536;      // bucket size of the LDS fast path (A and B)
 constexpr int kSub = 1024;          // sub-buckets of the LDS MSD split
 constexpr int kSubMax = 48;         // largest sub-bucket sorted by one thread
+constexpr uint32_t kWin = 8;        // sub-bucket entries a rank reads at once
 
 // Top-digit width: ~1K keys per bucket, between 1 and 11 bits.
 uint32_t digit_bits(uint32_t n) {
@@ -50,15 +51,29 @@ uint32_t digit_bits(uint32_t n) {
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Inclusive scan of a wave64: Hillis-Steele inside each row of 16 lanes with
+// row_shr DPP moves (1, 2, 4, 8), then the row totals carried across rows with
+// row_bcast:15 (rows 1 and 3) and row_bcast:31 (rows 2 and 3).  Lanes a move
+// does not reach receive 0.
+template <int CTRL, int ROWS>
+GPR_D uint32_t dpp_or_zero(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, false);
+}
+GPR_D uint32_t wave_scan_incl(uint32_t v) {
+    v += dpp_or_zero<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_or_zero<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_or_zero<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_or_zero<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_or_zero<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+    v += dpp_or_zero<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
+    return v;
+}
+
 // inclusive scan of one value per thread over a block of NT threads (wave64)
 template <int NT>
 GPR_D uint32_t scan_incl(uint32_t v, uint32_t* sh, uint32_t* total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(v, o, 64);
-        if (lane >= o) v += y;
-    }
+    v = wave_scan_incl(v);
     if (lane == 63) sh[wave] = v;
     __syncthreads();
     uint32_t add = 0, tot = 0;
@@ -107,7 +122,8 @@ __global__ __launch_bounds__(kTileThreads) void k_hist(const uint32_t* __restric
         uint32_t r = tot ? atomicAdd(&acc[d], tot) : 0u;
 #pragma unroll
         for (int x = 0; x < kQuarters; ++x) {
-            off[((uint64_t)blockIdx.x * kQuarters + x) * nd + d] = r;
+            __hip_atomic_store(&off[((uint64_t)blockIdx.x * kQuarters + x) * nd + d], r, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
             r += q[x];
         }
     }
@@ -212,7 +228,8 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __rest
         const uint32_t p = j * kTileThreads + threadIdx.x;
         if (p < m) {
             const unsigned long long w = stage[p];
-            tmp[gpos[(uint32_t)(w >> (32 + shift)) & (nd - 1)] + p] = w;
+            __hip_atomic_store(&tmp[gpos[(uint32_t)(w >> (32 + shift)) & (nd - 1)] + p], w, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -314,9 +331,16 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const u
             for (int j = 0; j < kPer; ++j) {
                 const uint32_t i = j * kLocalThreads + threadIdx.x;
                 if (i < n) {
+                    // rank among the sub-bucket: its first kWin entries are read
+                    // together (independent LDS loads); longer sub-buckets continue
                     const uint32_t b = start[sb[j]], c = cnt[sb[j]];
                     uint32_t rank = 0;
-                    for (uint32_t x = 0; x < c; ++x) rank += A[b + x] < v[j] ? 1u : 0u;
+#pragma unroll
+                    for (uint32_t x = 0; x < kWin; ++x) {
+                        const unsigned long long a = A[min(b + x, (uint32_t)kFastCap - 1)];
+                        rank += (x < c && a < v[j]) ? 1u : 0u;
+                    }
+                    for (uint32_t x = kWin; x < c; ++x) rank += A[b + x] < v[j] ? 1u : 0u;
                     B[b + rank] = v[j];
                 }
             }
@@ -326,8 +350,8 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const u
                 const uint32_t i = j * kLocalThreads + threadIdx.x;
                 if (i < n) {
                     const unsigned long long w = B[i];
-                    skeys[b0 + i] = (uint32_t)(w >> 32);
-                    sidx[b0 + i] = (uint32_t)w;
+                    __hip_atomic_store(&skeys[b0 + i], (uint32_t)(w >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&sidx[b0 + i], (uint32_t)w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
             return;
