@@ -96,6 +96,21 @@ GPR_HD void sincos_d(float x, double* s, double* c) {
 GPR_HD float sin_f(float x) { double s, c; sincos_d(x, &s, &c); return (float)s; }
 GPR_HD float cos_f(float x) { double s, c; sincos_d(x, &s, &c); return (float)c; }
 
+// GammaCorrect (spectrum.h:63-72) and the byte conversion of FrameBuffer::output
+// and filter_kernel (framebuffer.cpp:18-22, pathtracer_kernel.h:805-810):
+// (uint8)clamp(255 * gamma + 0.5, 0, 255), clamp mapping NaN to 255.  powf is
+// taken as a double pow rounded once (within one ulp of powf), so the host
+// writer and the device resolve give the same bytes.
+GPR_HD float gamma_correct(float v) {
+    if (v <= 0.0031308f) return 12.92f * v;
+    const float p = (float)pow((double)v, (double)(1.f / 2.4f));
+    return 1.055f * p - 0.055f;
+}
+GPR_HD uint32_t srgb_byte(float v) {
+    const float x = 255.f * gamma_correct(v) + 0.5f;
+    return (uint32_t)fmaxf(0.f, fminf(x, 255.f));
+}
+
 // ---------------------------------------------------------------- RNG ---
 // InitRandom (src/renderer/sampling.h:31-46): 16 TEA rounds, fixed key.
 GPR_HD uint32_t tea16(uint32_t v0, uint32_t v1) {

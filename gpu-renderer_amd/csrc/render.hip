@@ -1321,12 +1321,83 @@ int write_png(const std::string& path, uint32_t w, uint32_t h, const std::vector
     if (wr != out.size()) return fail(GPR_ERR_IO, "short write to " + path);
     return GPR_OK;
 }
-// GammaCorrect (spectrum.h:63-72)
-float gamma_correct(float v) {
-    if (v <= 0.0031308f) return 12.92f * v;
-    return 1.055f * std::pow(v, 1.f / 2.4f) - 0.055f;
+// Radiance RGBE of one pixel (Ward's float2rgbe): e from the largest channel,
+// mantissas scaled so that channel / 2^(e-8) fits a byte.  Non-finite or
+// negative channels are written as 0.
+void rgbe_of(const float* c, unsigned char* o) {
+    float v[3];
+    for (int k = 0; k < 3; ++k) v[k] = std::isfinite(c[k]) && c[k] > 0.f ? c[k] : 0.f;
+    const float m = std::max(v[0], std::max(v[1], v[2]));
+    if (m < 1e-32f) {
+        o[0] = o[1] = o[2] = o[3] = 0;
+        return;
+    }
+    int e;
+    const float scale = std::frexp(m, &e) * 256.f / m;
+    for (int k = 0; k < 3; ++k) o[k] = (unsigned char)std::min(255.f, v[k] * scale);
+    o[3] = (unsigned char)(e + 128);
+}
+// Radiance .hdr, 32-bit_rle_rgbe: new-style scanlines (each component as
+// literal dumps of up to 128 bytes) where the format allows them (8 <= w <
+// 32768), flat pixels otherwise; `rgb` rows top first.
+int write_hdr(const std::string& path, uint32_t w, uint32_t h, const std::vector<float>& rgb) {
+    std::string out = "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y " + std::to_string(h) + " +X " + std::to_string(w) + "\n";
+    std::vector<unsigned char> px(4ull * w);
+    for (uint32_t y = 0; y < h; ++y) {
+        for (uint32_t x = 0; x < w; ++x) rgbe_of(&rgb[3 * ((uint64_t)y * w + x)], &px[4 * x]);
+        if (w < 8 || w >= 32768) {
+            out.append(reinterpret_cast<const char*>(px.data()), px.size());
+            continue;
+        }
+        out += (char)2;
+        out += (char)2;
+        out += (char)(w >> 8);
+        out += (char)(w & 0xff);
+        for (int c = 0; c < 4; ++c)
+            for (uint32_t x0 = 0; x0 < w; x0 += 128) {
+                const uint32_t cnt = std::min<uint32_t>(128, w - x0);
+                out += (char)cnt;
+                for (uint32_t x = x0; x < x0 + cnt; ++x) out += (char)px[4 * x + c];
+            }
+    }
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return fail(GPR_ERR_IO, "cannot open " + path);
+    const size_t wr = std::fwrite(out.data(), 1, out.size(), f);
+    std::fclose(f);
+    if (wr != out.size()) return fail(GPR_ERR_IO, "short write to " + path);
+    return GPR_OK;
 }
 }  // namespace
+
+// filter_kernel (pathtracer_kernel.h:792-810): mean, sRGB bytes, alpha from
+// GammaCorrect(1); pixel i of the frame buffer to out[i] (R in the low byte).
+__global__ __launch_bounds__(256) void k_resolve_rgba8(const float4* __restrict__ fb, uint32_t npix, float n,
+                                                       uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= npix) return;
+    const float4 s = fb[i];
+    out[i] = srgb_byte(s.x / n) | (srgb_byte(s.y / n) << 8) | (srgb_byte(s.z / n) << 16) | (srgb_byte(1.f) << 24);
+}
+
+extern "C" int gpr_renderer_resolve_rgba8(gpr_renderer* r, uint32_t* out, uint32_t flags) {
+    if (!r || !out) return fail(GPR_ERR_INVALID, "gpr_renderer_resolve_rgba8: null argument");
+    if (flags & ~GPR_RESOLVE_DEVICE_PTRS) return fail(GPR_ERR_INVALID, "gpr_renderer_resolve_rgba8: unknown flags");
+    const float n = (float)std::max(1u, r->spp_done);
+    const uint32_t grid = (uint32_t)((r->npix + 255) / 256);
+    if (flags & GPR_RESOLVE_DEVICE_PTRS) {
+        hipLaunchKernelGGL(k_resolve_rgba8, dim3(grid), dim3(256), 0, stream(), r->fb.as<float4>(), r->npix, n, out);
+        GPR_HIP(hipGetLastError());
+        return GPR_OK;
+    }
+    DevBuf d;
+    GPR_TRY(d.alloc(4ull * r->npix));
+    hipLaunchKernelGGL(k_resolve_rgba8, dim3(grid), dim3(256), 0, stream(), r->fb.as<float4>(), r->npix, n,
+                       d.as<uint32_t>());
+    GPR_HIP(hipGetLastError());
+    GPR_HIP(hipMemcpyAsync(out, d.p, 4ull * r->npix, hipMemcpyDeviceToHost, stream()));
+    GPR_HIP(hipStreamSynchronize(stream()));
+    return GPR_OK;
+}
 
 extern "C" int gpr_renderer_write_png(gpr_renderer* r, const char* path) {
     if (!r || !path) return fail(GPR_ERR_INVALID, "gpr_renderer_write_png: null argument");
@@ -1340,11 +1411,27 @@ extern "C" int gpr_renderer_write_png(gpr_renderer* r, const char* path) {
     for (uint32_t y = 0; y < h; ++y)
         for (uint32_t x = 0; x < w; ++x) {
             const uint64_t src = (uint64_t)y * w + x, dst = (uint64_t)(h - 1 - y) * w + x;
-            for (int c = 0; c < 3; ++c) {
-                float v = 255.f * gamma_correct(rgb[3 * src + c] / n) + 0.5f;
-                v = v < 0.f ? 0.f : (v > 255.f ? 255.f : v);
-                px[3 * dst + c] = (unsigned char)v;
-            }
+            for (int c = 0; c < 3; ++c) px[3 * dst + c] = (unsigned char)srgb_byte(rgb[3 * src + c] / n);
         }
     return write_png(path, w, h, px);
+}
+
+// WriteImage (imageio.cpp:59-82) chosen by the file extension.
+extern "C" int gpr_renderer_write_image(gpr_renderer* r, const char* path) {
+    if (!r || !path) return fail(GPR_ERR_INVALID, "gpr_renderer_write_image: null argument");
+    std::string p(path), ext;
+    const size_t dot = p.find_last_of('.');
+    if (dot != std::string::npos) ext = p.substr(dot);
+    for (char& ch : ext) ch = (char)std::tolower((unsigned char)ch);
+    if (ext == ".png") return gpr_renderer_write_png(r, path);
+    if (ext != ".hdr") return fail(GPR_ERR_UNSUPPORTED, "gpr_renderer_write_image: extension must be .png or .hdr");
+    const uint32_t w = r->scene->width, h = r->scene->height;
+    std::vector<float> rgb(3ull * r->npix), mean(3ull * r->npix);
+    GPR_TRY(gpr_renderer_read(r, rgb.data(), nullptr));
+    const float n = (float)std::max(1u, r->spp_done);
+    for (uint32_t y = 0; y < h; ++y)  // rows flipped on write, as for the PNG
+        for (uint32_t x = 0; x < w; ++x)
+            for (int c = 0; c < 3; ++c)
+                mean[3 * ((uint64_t)(h - 1 - y) * w + x) + c] = rgb[3 * ((uint64_t)y * w + x) + c] / n;
+    return write_hdr(path, w, h, mean);
 }

@@ -40,6 +40,7 @@ MATERIAL_DTYPE = np.dtype([("type", "<u4"), ("color", "<f4", 3), ("color1", "<f4
 assert MATERIAL_DTYPE.itemsize == C.sizeof(Material)
 # texture slots of Material.tex (a handle from Scene.add_texture; 0 = the constant field)
 TEX_COLOR, TEX_COLOR1, TEX_IOR, TEX_ALPHA_X, TEX_ALPHA_Y, TEX_ETA_I, TEX_ETA_T, TEX_K = range(8)
+RESOLVE_DEVICE_PTRS = 1
 TEXTURE_DTYPE = np.dtype([("offset", "<u4"), ("width", "<u4"), ("height", "<u4"), ("pad", "<u4"),
                           ("uv_offset", "<f4", 2), ("uv_scale", "<f4", 2)])
 
@@ -162,6 +163,8 @@ _SIG = {
     "gpr_render": (C.c_int, [_P, C.c_uint32]),
     "gpr_renderer_read": (C.c_int, [_P, _P, _P]),
     "gpr_renderer_write_png": (C.c_int, [_P, C.c_char_p]),
+    "gpr_renderer_write_image": (C.c_int, [_P, C.c_char_p]),
+    "gpr_renderer_resolve_rgba8": (C.c_int, [_P, _P, C.c_uint32]),
     "gpr_renderer_reset": (C.c_int, [_P]),
     "gpr_renderer_queue_sizes": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "gpr_renderer_free": (C.c_int, [_P]),
@@ -508,6 +511,20 @@ class Renderer:
 
     def write_png(self, path):
         check(lib.gpr_renderer_write_png(self.h, os.fsencode(path)))
+
+    def write_image(self, path):
+        """.png or .hdr by extension"""
+        check(lib.gpr_renderer_write_image(self.h, os.fsencode(path)))
+
+    def resolve_rgba8(self, device_ptr=None):
+        """RGBA8 of the mean (frame-buffer order).  With device_ptr the bytes go to
+        that device buffer (no synchronisation) and None is returned."""
+        if device_ptr is not None:
+            check(lib.gpr_renderer_resolve_rgba8(self.h, C.c_void_p(device_ptr), RESOLVE_DEVICE_PTRS))
+            return None
+        out = np.empty(self.w * self.h_px, np.uint32)
+        check(lib.gpr_renderer_resolve_rgba8(self.h, _ptr(out), 0))
+        return out
 
     def reset(self):
         check(lib.gpr_renderer_reset(self.h))

@@ -290,6 +290,18 @@ int gpr_render(gpr_renderer* r, uint32_t spp);
 int gpr_renderer_read(gpr_renderer* r, float* rgb_sum, uint32_t* sample_num);
 /* FrameBuffer::output (framebuffer.cpp:11-26): mean, sRGB gamma, 8-bit PNG. */
 int gpr_renderer_write_png(gpr_renderer* r, const char* path);
+/* WriteImage (imageio.cpp:59-82) by extension: ".png" as gpr_renderer_write_png,
+ * ".hdr" a Radiance RGBE file of the mean linear radiance, rows flipped like the
+ * PNG.  (The reference's ".hdr" branch hands its 8-bit buffer to stbi_write_hdr
+ * as floats; this writes the radiance that buffer stands for.) */
+int gpr_renderer_write_image(gpr_renderer* r, const char* path);
+#define GPR_RESOLVE_DEVICE_PTRS 1u
+/* filter / filter_kernel (pathtracer_kernel.h:792-821): per pixel mean -> sRGB
+ * -> RGBA8 (alpha 255) into out, width*height uint32 in frame-buffer order (no
+ * flip), R in the low byte.  With GPR_RESOLVE_DEVICE_PTRS out is device memory
+ * (a display pixel buffer) and the call returns without synchronising; else out
+ * is host memory. */
+int gpr_renderer_resolve_rgba8(gpr_renderer* r, uint32_t* out, uint32_t flags);
 /* PathTracer::reset (pathtracer.cu:195-200). */
 int gpr_renderer_reset(gpr_renderer* r);
 /* Interactive camera and frame operations of PathTracer (pathtracer.cu:177-209):

@@ -140,6 +140,62 @@ def test_free_camera_and_interactive_ops_bit_exact():
     r.free()
 
 
+def _png_rgb(path):
+    """Pixels of an 8-bit RGB PNG (any zlib stream): rows top first."""
+    import struct
+    import zlib
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    p, idat, w, h = 8, b"", 0, 0
+    while p < len(data):
+        ln = struct.unpack(">I", data[p:p + 4])[0]
+        t = data[p + 4:p + 8]
+        if t == b"IHDR":
+            w, h = struct.unpack(">II", data[p + 8:p + 16])
+        elif t == b"IDAT":
+            idat += data[p + 8:p + 8 + ln]
+        p += 12 + ln
+    raw = zlib.decompress(idat)
+    rows = [raw[y * (3 * w + 1):(y + 1) * (3 * w + 1)] for y in range(h)]
+    assert all(r[0] == 0 for r in rows)  # filter type None
+    return np.frombuffer(b"".join(r[1:] for r in rows), np.uint8).reshape(h, w, 3)
+
+
+def test_output_path_png_hdr_and_rgba8(tmp_path):
+    # filter_kernel bytes (device resolve), FrameBuffer::output (PNG, rows
+    # flipped) and the .hdr writer, all from the same accumulation
+    import torch
+    s = gpr.Scene.cornell(40, 24)
+    r = gpr.Renderer(s, max_depth=4, rng_mode=gpr.RNG_PCG, rr_start=3, seed=3)
+    r.render(3)
+    rgb, cnt = r.read()
+    mean = (rgb / cnt[:, None].astype(np.float32)).reshape(24, 40, 3)
+    host = r.resolve_rgba8()
+    dev = torch.zeros(40 * 24, dtype=torch.int32, device="cuda")
+    r.resolve_rgba8(dev.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(dev.cpu().numpy().view(np.uint32), host)
+    b = host.view(np.uint8).reshape(24, 40, 4)
+    assert np.all(b[:, :, 3] == 255)
+    # sRGB bytes against the closed form (within one step where powf rounding may tip a byte)
+    lin = mean.astype(np.float64)
+    srgb = np.where(lin <= 0.0031308, 12.92 * lin, 1.055 * np.power(np.maximum(lin, 0), 1 / 2.4) - 0.055)
+    want = np.clip(np.floor(255 * srgb + 0.5), 0, 255)
+    assert np.abs(b[:, :, :3].astype(np.int64) - want).max() <= 1
+    assert (b[:, :, :3] != want).mean() < 0.01
+    r.write_image(str(tmp_path / "o.png"))
+    png = _png_rgb(str(tmp_path / "o.png"))
+    assert np.array_equal(png, b[::-1, :, :3])
+    r.write_image(str(tmp_path / "o.hdr"))
+    hdr = gpr.read_image(str(tmp_path / "o.hdr"))  # read flips back: frame-buffer order
+    assert hdr.shape == (24, 40, 3)
+    m = np.maximum(mean.max(axis=2, keepdims=True), 1e-30)
+    assert np.all(np.abs(hdr - mean) <= m / 128.0 + 1e-30)  # RGBE keeps 8 bits under the largest channel
+    with pytest.raises(gpr.GprError):
+        r.write_image(str(tmp_path / "o.jpg"))
+    r.free()
+
+
 def test_reference_cbox_scene_file_bit_exact():
     # The reference's own scene file (scene/CBox/cbox.xml: OBJ spheres with
     # mirror and glass, Lambert walls, area light) through the loader, at its
