@@ -98,6 +98,7 @@ M4 m_inverse(const M4& a) {
 struct Xf {
     M4 m, inv;
 };
+}  // namespace
 // Inverse of a row-major 4x4; false when it is not finite (singular input).
 bool invert4(const float a[16], float out[16]) {
     M4 m;
@@ -108,6 +109,7 @@ bool invert4(const float a[16], float out[16]) {
         if (!std::isfinite(out[k])) return false;
     return true;
 }
+namespace {
 Xf x_mul(const Xf& a, const Xf& b) { return Xf{m_mul(a.m, b.m), m_mul(b.inv, a.inv)}; }
 Xf x_scale(float x, float y, float z) {
     Xf t{m_ident(), m_ident()};

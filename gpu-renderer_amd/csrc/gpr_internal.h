@@ -191,6 +191,9 @@ int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64
 bool trace_cfg_valid(int nt, int sl, int chunk);
 // imageio.cpp: ReadImage -- width*height*3 linear floats, bottom row first
 int read_image(const std::string& path, uint32_t* w, uint32_t* h, std::vector<float>* rgb);
+// Gauss-Jordan inverse of a row-major 4x4 (transform.cpp:30-90); false when
+// the result is not finite
+bool invert4(const float a[16], float out[16]);
 // host math shared by api.cpp and the renderer
 void camera_raster_to_camera(uint32_t w, uint32_t h, float fov_deg, float out[16]);
 void lookat_matrix(const float target[3], const float origin[3], const float up[3], float m[16]);

@@ -178,43 +178,11 @@ M4 mul(const M4& a, const M4& b) {
             r.m[p][q] = a.m[p][0] * b.m[0][q] + a.m[p][1] * b.m[1][q] + a.m[p][2] * b.m[2][q] + a.m[p][3] * b.m[3][q];
     return r;
 }
-// Gauss-Jordan with full pivoting (transform.cpp:30-90)
+// Inverse (transform.cpp:30-90): the Gauss-Jordan elimination shared with
+// the scene API (api.cpp).
 M4 inverse(const M4& a) {
-    int cols[4], rows[4], used[4] = {0, 0, 0, 0};
-    float w[4][4];
-    std::memcpy(w, a.m, sizeof(w));
-    for (int it = 0; it < 4; ++it) {
-        int pr = 0, pc = 0;
-        float best = 0.f;
-        for (int r = 0; r < 4; ++r) {
-            if (used[r] == 1) continue;
-            for (int c = 0; c < 4; ++c)
-                if (used[c] == 0 && std::fabs(w[r][c]) >= best) {
-                    best = std::fabs(w[r][c]);
-                    pr = r;
-                    pc = c;
-                }
-        }
-        ++used[pc];
-        if (pr != pc)
-            for (int k = 0; k < 4; ++k) std::swap(w[pr][k], w[pc][k]);
-        rows[it] = pr;
-        cols[it] = pc;
-        const float inv = 1.f / w[pc][pc];
-        w[pc][pc] = 1.f;
-        for (int k = 0; k < 4; ++k) w[pc][k] *= inv;
-        for (int r = 0; r < 4; ++r) {
-            if (r == pc) continue;
-            const float f = w[r][pc];
-            w[r][pc] = 0;
-            for (int k = 0; k < 4; ++k) w[r][k] -= w[pc][k] * f;
-        }
-    }
-    for (int it = 3; it >= 0; --it)
-        if (rows[it] != cols[it])
-            for (int k = 0; k < 4; ++k) std::swap(w[k][rows[it]], w[k][cols[it]]);
     M4 r;
-    std::memcpy(r.m, w, sizeof(w));
+    (void)invert4(&a.m[0][0], &r.m[0][0]);
     return r;
 }
 struct Xform {

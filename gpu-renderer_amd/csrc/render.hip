@@ -537,7 +537,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
             const Geom g = setup_geom(sv, t, hh.z, hh.w);
             const v3 wo = -norm3(rd);
             gpr_material m = sv.mats[sv.mat[t]];
-            if (has_textures(m)) resolve_textures(sv, m, t, hh.z, hh.w);
+            if (sv.ntex != 0 && has_textures(m)) resolve_textures(sv, m, t, hh.z, hh.w);
             const Frame fr = make_frame(g.ns);
             if (bounce == 0 || spec) {
                 const c3 le = loadc(m.emission);
