@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--scenes", type=int, default=0, help="scenes of the batch / serve workloads (default 256 / 64)")
     ap.add_argument("--no-serve", action="store_true", help="raycast run without the served-stream segment")
     ap.add_argument("--capacity", type=int, default=1 << 24, help="paths per pass of the served / batched engine")
-    ap.add_argument("--trace-cfg", default="128,16,64,1",
+    ap.add_argument("--trace-cfg", default="64,16,64,1",
                     help="block,stack,chunk,refill of the --only trace profiling run (the tuned configuration)")
     return ap.parse_args()
 

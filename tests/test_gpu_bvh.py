@@ -62,7 +62,8 @@ def test_build_1m_soup():
 def test_sort_device_entry_point():
     import torch
     rng = np.random.default_rng(4)
-    for n in (1, 5, 4096, 4097, 300001):
+    # 4M keys: 2048 top-digit buckets of ~2K keys, above the bucket pass's LDS fast path
+    for n in (1, 5, 4096, 4097, 300001, 1 << 22):
         keys = rng.integers(0, 1 << 30, n).astype(np.uint32)
         keys[: n // 3] = keys[0]  # a long run of equal keys
         dk = torch.from_numpy(keys.view(np.int32)).cuda()
