@@ -164,6 +164,7 @@ GPR_D void tri_rec_hit(const TriRec* tris, uint32_t ntri, int32_t leaf, const Ra
 struct Trav {
     Ray3 r;
     v3 inv;
+    int sx, sy, sz;  // 1 where the direction component is negative: the far plane comes first
     float tmax;
     gpr_hit hit;
     int32_t node;
@@ -187,6 +188,9 @@ GPR_D bool trav_begin(const SceneView& sv, const Ray3& r, Stack& st, Trav& t) {
         return true;
     }
     t.inv = V(1 / r.d.x, 1 / r.d.y, 1 / r.d.z);
+    t.sx = __builtin_signbit(t.inv.x) ? 1 : 0;
+    t.sy = __builtin_signbit(t.inv.y) ? 1 : 0;
+    t.sz = __builtin_signbit(t.inv.z) ? 1 : 0;
     t.node = sv.root4;
     return false;
 }
@@ -197,21 +201,27 @@ GPR_D int trav_step(const SceneView& sv, Stack& st, Trav& t) {
     if ((uint32_t)t.node >= sv.nbvh4 || t.fetched > sv.nbvh4) return kTravOverflow;
     const Ray3& r = t.r;
     const float4* np = reinterpret_cast<const float4*>(sv.bvh4 + t.node);
-    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+    // the ray enters each slab at the plane its direction meets first: near
+    // and far planes are loaded by the direction's signs (lx/hx at np[0]/np[1],
+    // ly/hy at np[2]/np[3], lz/hz at np[4]/np[5]), so no per-axis min / max.
+    // A 0 * inf (ray in a slab plane) gives NaN, which fmaxf / fminf ignore:
+    // that axis then does not cull the child.
+    const float4 nx = np[0 + t.sx], fx = np[1 - t.sx], ny = np[2 + t.sy], fy = np[3 - t.sy];
+    const float4 nz = np[4 + t.sz], fz = np[5 - t.sz];
     const int4 ch = *reinterpret_cast<const int4*>(np + 6);
-    const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+    const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
     const int32_t cha[4] = {ch.x, ch.y, ch.z, ch.w};
     float tn[4];
     bool hitc[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float x0 = (lxa[c] - r.o.x) * t.inv.x, x1 = (hxa[c] - r.o.x) * t.inv.x;
-        const float y0 = (lya[c] - r.o.y) * t.inv.y, y1 = (hya[c] - r.o.y) * t.inv.y;
-        const float z0 = (lza[c] - r.o.z) * t.inv.z, z1 = (hza[c] - r.o.z) * t.inv.z;
-        const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
-        const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), t.tmax));
+        const float x0 = (nxa[c] - r.o.x) * t.inv.x, x1 = (fxa[c] - r.o.x) * t.inv.x;
+        const float y0 = (nya[c] - r.o.y) * t.inv.y, y1 = (fya[c] - r.o.y) * t.inv.y;
+        const float z0 = (nza[c] - r.o.z) * t.inv.z, z1 = (fza[c] - r.o.z) * t.inv.z;
+        const float tnear = fmaxf(fmaxf(x0, y0), fmaxf(z0, r.tmin));
+        const float tfar = fminf(fminf(x1, y1), fminf(z1, t.tmax));
         tn[c] = tnear;
         hitc[c] = tnear <= tfar * kSlabCons && cha[c] != kEmptyChild;
     }
