@@ -1194,6 +1194,23 @@ int gpr_renderer_reset(gpr_renderer* r) {
     return GPR_OK;
 }
 
+int gpr_renderer_restore(gpr_renderer* r, const float* rgb, uint32_t spp) {
+    if (!r || !rgb) return fail(GPR_ERR_INVALID, "gpr_renderer_restore: null argument");
+    if (r->opts.rng_mode != GPR_RNG_PCG)
+        return fail(GPR_ERR_UNSUPPORTED,
+                    "gpr_renderer_restore: reference RNG mode keeps slot-indexed seeds across samples; "
+                    "only PCG renders can be continued");
+    std::vector<float4> tmp(r->npix);
+    for (uint64_t i = 0; i < r->npix; ++i) tmp[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.f);
+    GPR_HIP(hipMemcpyAsync(r->fb.p, tmp.data(), 16ull * r->npix, hipMemcpyHostToDevice, stream()));
+    hipLaunchKernelGGL(k_set_sample, dim3(1), dim3(64), 0, stream(), r->ctrl.as<Ctrl>(), spp);
+    GPR_HIP(hipGetLastError());
+    GPR_HIP(hipStreamSynchronize(stream()));  // tmp is released on return
+    r->spp_done = spp;
+    r->reset_pending = false;
+    return GPR_OK;
+}
+
 int gpr_render(gpr_renderer* r, uint32_t spp) {
     if (!r) return fail(GPR_ERR_INVALID, "gpr_render: null renderer");
     if (r->scene->bvh.n == 0) return fail(GPR_ERR_UNSUPPORTED, "gpr_render: scene has no triangles");

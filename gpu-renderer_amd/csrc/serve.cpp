@@ -376,7 +376,9 @@ struct Request {
     gpr_job job{};
     EngineJob ej{};
     uint32_t issued = 0;
-    int state = 0;  // 0 pending, 1 active, 2 done, 3 failed
+    int state = 0;  // 0 pending, 1 active, 2 done, 3 failed, 4 cancelled
+    bool cancel = false;    // cancellation requested while active
+    bool retiring = false;  // no more samples; the range is released after the passes in flight
     double t_submit = 0, t_done = 0;
     std::vector<float> rgb;
 };
@@ -401,11 +403,12 @@ struct gpr_server {
     bool stop = false;
     bool scenes_dirty = false;
     bool in_pass = false;  // the loop is launching a pass outside the lock
+    bool cancel_req = false;  // an active request asked to be cancelled
     int loop_error = GPR_OK;
     std::string loop_msg;
     std::thread loop;
     // statistics
-    uint64_t submitted = 0, completed = 0, passes = 0, jobs_in_passes = 0;
+    uint64_t submitted = 0, completed = 0, passes = 0, jobs_in_passes = 0, cancelled = 0;
     double t_first_submit = 0, t_last_done = 0;
     std::vector<double> latencies;
     ~gpr_server() {
@@ -419,6 +422,7 @@ namespace {
 struct Finishing {
     hipEvent_t ev;
     std::vector<Request*> reqs;
+    bool cancelled = false;  // retire the requests (no result) instead of copying them out
 };
 
 // Admission: FIFO, a request enters when its frame buffer fits in the pool
@@ -449,6 +453,21 @@ void loop_fail(gpr_server* s, const std::string& msg) {
 // Copy the finished requests' frame buffers out, release their pool ranges.
 bool drain(gpr_server* s, Finishing& f) {
     if (hipStreamWaitEvent(s->cst, f.ev, 0) != hipSuccess) return false;
+    if (f.cancelled) {
+        // the passes that could still write into these ranges have completed
+        if (hipStreamSynchronize(s->cst) != hipSuccess) return false;
+        std::lock_guard<std::mutex> g(s->mu);
+        for (Request* r : f.reqs) {
+            s->pool.give(r->ej.fb_off, r->ej.npix);
+            r->state = 4;
+            s->cancelled++;
+            s->active.erase(std::remove(s->active.begin(), s->active.end(), r), s->active.end());
+        }
+        (void)hipEventDestroy(f.ev);
+        s->done_cv.notify_all();
+        s->cv.notify_all();
+        return true;
+    }
     std::vector<float> tmp;
     for (Request* r : f.reqs) {
         tmp.resize(4ull * r->ej.npix);
@@ -485,6 +504,7 @@ void server_loop(gpr_server* s) {
     std::deque<Finishing> finishing;
     std::vector<EngineJob> pass;
     std::vector<Request*> pass_reqs;
+    std::vector<Request*> retire;
     while (true) {
         TraceCfg cfg;
         {
@@ -493,7 +513,7 @@ void server_loop(gpr_server* s) {
                 admit(s);
                 bool work = false;
                 for (Request* r : s->active) work = work || r->issued < r->job.spp;
-                if (work || !finishing.empty() || s->scenes_dirty) break;
+                if (work || !finishing.empty() || s->scenes_dirty || s->cancel_req) break;
                 if (s->stop && s->pending.empty() && s->active.empty()) return;
                 s->cv.wait(lk);
             }
@@ -506,6 +526,16 @@ void server_loop(gpr_server* s) {
                 s->scenes_dirty = false;
             }
             cfg = s->cfg;
+            // cancelled requests still owed samples stop here; the ones whose
+            // last sample is already issued finish normally
+            retire.clear();
+            for (Request* r : s->active)
+                if (r->cancel && !r->retiring && r->issued < r->job.spp) {
+                    r->retiring = true;
+                    r->issued = r->job.spp;
+                    retire.push_back(r);
+                }
+            s->cancel_req = false;
             // pack: active requests in admission order, one sample each
             pass.clear();
             pass_reqs.clear();
@@ -547,6 +577,18 @@ void server_loop(gpr_server* s) {
                 }
                 finishing.push_back(std::move(f));
             }
+        }
+        if (!retire.empty()) {
+            // released once every pass launched so far (the last one that may
+            // have included them) has completed
+            Finishing f{};
+            f.reqs = retire;
+            f.cancelled = true;
+            if (hipEventCreateWithFlags(&f.ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(f.ev, s->st) != hipSuccess) {
+                loop_fail(s, "server: event failed");
+                return;
+            }
+            finishing.push_back(std::move(f));
         }
         // keep one pass queued ahead of the completions being drained
         while (finishing.size() > (pass.empty() ? 0u : 1u)) {
@@ -632,9 +674,33 @@ int gpr_server_wait(gpr_server* s, uint64_t ticket, float* rgb, double* latency_
         s->reqs.erase(it);
         return fail(GPR_ERR_HIP, "gpr_server_wait: " + s->loop_msg);
     }
+    if (r->state == 4) {
+        s->reqs.erase(it);
+        return fail(GPR_ERR_CANCELLED, "gpr_server_wait: request " + std::to_string(ticket) + " was cancelled");
+    }
     if (rgb) std::memcpy(rgb, r->rgb.data(), sizeof(float) * r->rgb.size());
     if (latency_ms) *latency_ms = r->t_done - r->t_submit;
     s->reqs.erase(it);
+    return GPR_OK;
+}
+
+int gpr_server_cancel(gpr_server* s, uint64_t ticket) {
+    if (!s) return fail(GPR_ERR_INVALID, "gpr_server_cancel: null server");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(GPR_ERR_INVALID, "gpr_server_cancel: unknown ticket");
+    Request* r = it->second.get();
+    if (r->state == 0) {
+        s->pending.erase(std::remove(s->pending.begin(), s->pending.end(), r), s->pending.end());
+        r->state = 4;
+        s->cancelled++;
+        s->done_cv.notify_all();
+        s->cv.notify_all();  // the queue head may have changed
+    } else if (r->state == 1 && !r->cancel) {
+        r->cancel = true;
+        s->cancel_req = true;
+        s->cv.notify_all();
+    }
     return GPR_OK;
 }
 
@@ -689,6 +755,7 @@ int gpr_server_get_stats(gpr_server* s, gpr_server_stats* out) {
     o.completed = s->completed;
     o.passes = s->passes;
     o.mean_jobs_per_pass = s->passes ? (double)s->jobs_in_passes / s->passes : 0.0;
+    o.cancelled = s->cancelled;
     if (s->completed) {
         const double span = s->t_last_done - s->t_first_submit;
         o.renders_per_s = span > 0 ? 1000.0 * s->completed / span : 0.0;

@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgpr.so")
 
 OK = 0
-ERR_INVALID, ERR_OOM, ERR_HIP, ERR_IO, ERR_PARSE, ERR_UNSUPPORTED, ERR_BUDGET = -1, -2, -3, -4, -5, -6, -7
+ERR_INVALID, ERR_OOM, ERR_HIP, ERR_IO, ERR_PARSE, ERR_UNSUPPORTED, ERR_BUDGET, ERR_CANCELLED = -1, -2, -3, -4, -5, -6, -7, -8
 
 MAT_DIFFUSE, MAT_SPECULAR, MAT_ROUGH_CONDUCTOR, MAT_ROUGH_DIELECTRIC, MAT_ROUGH_PLASTIC = 0, 1, 2, 3, 4
 TRACE_ANY, TRACE_REF_ORDER, TRACE_DEVICE_PTRS = 1, 2, 4
@@ -93,7 +93,7 @@ class ServerOpts(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("passes", C.c_uint64),
                 ("renders_per_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
-                ("mean_jobs_per_pass", C.c_double)]
+                ("mean_jobs_per_pass", C.c_double), ("cancelled", C.c_uint64)]
 
 
 class GprError(RuntimeError):
@@ -164,6 +164,7 @@ _SIG = {
     "gpr_renderer_read": (C.c_int, [_P, _P, _P]),
     "gpr_renderer_write_png": (C.c_int, [_P, C.c_char_p]),
     "gpr_renderer_write_image": (C.c_int, [_P, C.c_char_p]),
+    "gpr_renderer_restore": (C.c_int, [_P, _P, C.c_uint32]),
     "gpr_renderer_resolve_rgba8": (C.c_int, [_P, _P, C.c_uint32]),
     "gpr_renderer_reset": (C.c_int, [_P]),
     "gpr_renderer_queue_sizes": (C.c_int, [_P, _P, _P, C.c_uint32]),
@@ -182,6 +183,7 @@ _SIG = {
     "gpr_server_add_scene": (C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
     "gpr_server_submit": (C.c_int, [_P, C.POINTER(Job), C.POINTER(C.c_uint64)]),
     "gpr_server_wait": (C.c_int, [_P, C.c_uint64, _P, C.POINTER(C.c_double)]),
+    "gpr_server_cancel": (C.c_int, [_P, C.c_uint64]),
     "gpr_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "gpr_server_tune": (C.c_int, [_P, C.POINTER(Job), C.c_uint32, C.POINTER(TraceCfg), C.c_uint32,
                                   C.POINTER(C.c_uint32)]),
@@ -512,6 +514,13 @@ class Renderer:
     def write_png(self, path):
         check(lib.gpr_renderer_write_png(self.h, os.fsencode(path)))
 
+    def restore(self, rgb_sum, spp):
+        """continue from a checkpoint (rgb_sum as read() returns it, spp samples)"""
+        a = np.ascontiguousarray(rgb_sum, np.float32).reshape(-1)
+        if a.size != 3 * self.w * self.h_px:
+            raise ValueError("checkpoint size does not match the image")
+        check(lib.gpr_renderer_restore(self.h, _ptr(a), spp))
+
     def write_image(self, path):
         """.png or .hdr by extension"""
         check(lib.gpr_renderer_write_image(self.h, os.fsencode(path)))
@@ -662,6 +671,9 @@ class Server:
         check(lib.gpr_server_submit(self.h, C.byref(j), C.byref(t)))
         self.sizes[t.value] = j.width * j.height
         return t.value
+
+    def cancel(self, ticket):
+        check(lib.gpr_server_cancel(self.h, ticket))
 
     def wait(self, ticket, want_image=True):
         n = self.sizes.pop(ticket)

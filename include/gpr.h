@@ -31,6 +31,7 @@ extern "C" {
 #define GPR_ERR_PARSE -5       /* scene file malformed */
 #define GPR_ERR_UNSUPPORTED -6 /* feature outside the supported subset */
 #define GPR_ERR_BUDGET -7      /* request exceeds the device-memory budget */
+#define GPR_ERR_CANCELLED -8   /* served request cancelled before it finished */
 
 /* Ray: origin, direction, [tmin, tmax] -- Ray (src/renderer/ray.h:5-11), 32 B. */
 typedef struct {
@@ -288,6 +289,15 @@ int gpr_renderer_create(gpr_scene* s, const gpr_render_opts* opts, gpr_renderer*
 int gpr_render(gpr_renderer* r, uint32_t spp);
 /* Linear radiance sum and per-pixel sample count (FrameBuffer, framebuffer.h:33-70). */
 int gpr_renderer_read(gpr_renderer* r, float* rgb_sum, uint32_t* sample_num);
+/* Continue a progressive render from a checkpoint: rgb_sum (width*height*3
+ * linear radiance sums, as gpr_renderer_read returns them) holding `spp`
+ * samples.  The next gpr_render renders sample `spp` onward; in PCG mode the
+ * result is bit-identical to an uninterrupted render, since every sample's
+ * random numbers depend on the seed, the sample index and the pixel only.
+ * Reference RNG mode keeps slot-indexed seeds across samples and is refused.
+ * No counterpart in the reference (it writes its buffer once, pathtracer.cu:168);
+ * for progressive and served renders that are saved and resumed. */
+int gpr_renderer_restore(gpr_renderer* r, const float* rgb_sum, uint32_t spp);
 /* FrameBuffer::output (framebuffer.cpp:11-26): mean, sRGB gamma, 8-bit PNG. */
 int gpr_renderer_write_png(gpr_renderer* r, const char* path);
 /* WriteImage (imageio.cpp:59-82) by extension: ".png" as gpr_renderer_write_png,
@@ -390,6 +400,12 @@ int gpr_server_submit(gpr_server* s, const gpr_job* job, uint64_t* ticket);
  * (width*height*3 floats, may be NULL) and its latency (submit -> result
  * readable, ms, may be NULL).  Each ticket is waited once. */
 int gpr_server_wait(gpr_server* s, uint64_t ticket, float* rgb_sum, double* latency_ms);
+/* Cancel a request: a queued one leaves the queue; an admitted one receives no
+ * further samples and its frame-buffer range returns to the pool once the passes
+ * already launched with it have completed.  gpr_server_wait then returns
+ * GPR_ERR_CANCELLED.  A request that has already finished keeps its result.
+ * No counterpart in the reference (it has no request loop). */
+int gpr_server_cancel(gpr_server* s, uint64_t ticket);
 /* Autotune the server's traversal configuration on probe jobs (one pass of
  * their first sample per configuration, as gpr_batch_tune); only while no
  * request is queued or in flight.  Best first in out[0..n), n <= cap. */
@@ -402,6 +418,7 @@ typedef struct {
     double renders_per_s;  /* completed / (last completion - first submit) */
     double p50_ms, p99_ms; /* latency percentiles of the completed requests */
     double mean_jobs_per_pass;
+    uint64_t cancelled;    /* requests cancelled before they finished */
 } gpr_server_stats;
 int gpr_server_get_stats(gpr_server* s, gpr_server_stats* out);
 /* Stops the loop after the requests in flight are done, frees everything. */

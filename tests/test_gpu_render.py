@@ -196,6 +196,31 @@ def test_output_path_png_hdr_and_rgba8(tmp_path):
     r.free()
 
 
+def test_checkpoint_restore_continues_bit_exact():
+    # render 4 samples at once, or 1 + checkpoint + restore into a new renderer + 3
+    s = gpr.Scene.cornell(32, 24)
+    kw = dict(max_depth=5, rng_mode=gpr.RNG_PCG, rr_start=2, seed=21)
+    r = gpr.Renderer(s, **kw)
+    r.render(4)
+    full, cnt = r.read()
+    r.free()
+    r1 = gpr.Renderer(s, **kw)
+    r1.render(1)
+    ck, _ = r1.read()
+    r1.free()
+    r2 = gpr.Renderer(s, **kw)
+    r2.restore(ck, 1)
+    r2.render(3)
+    cont, cnt2 = r2.read()
+    assert np.array_equal(cnt2, cnt)
+    assert np.array_equal(cont.view(np.uint32), full.view(np.uint32))
+    r2.free()
+    rr = gpr.Renderer(s, max_depth=5, rng_mode=gpr.RNG_REFERENCE, rr_start=2, seed=21)
+    with pytest.raises(gpr.GprError):
+        rr.restore(ck, 1)
+    rr.free()
+
+
 def test_reference_cbox_scene_file_bit_exact():
     # The reference's own scene file (scene/CBox/cbox.xml: OBJ spheres with
     # mirror and glass, Lambert walls, area light) through the loader, at its

@@ -154,3 +154,36 @@ def test_server_budget_admission(scenes):
         srv.submit(gpr.job(0, 1 << 15, 1 << 15, 1, cam))
     assert e.value.code in (gpr.ERR_INVALID, gpr.ERR_BUDGET)
     srv.free()
+
+
+def test_server_cancel_queued_active_and_finished(scenes):
+    # one job slot: the second request stays queued while the first renders
+    import time
+    arena = sum(gpr.arena_bytes(x) for x in scenes)
+    srv = gpr.Server(arena + (1 << 30), arena, capacity=4096, max_jobs=1)
+    ids = [srv.add_scene(x) for x in scenes]
+    cam = gpr.lookat((0, 0, 0), (0.3, 0.1, 3.9))
+    long_job = gpr.job(ids[0], 48, 40, 4000, cam, fov=40, max_depth=5, rr_start=3, seed=1)
+    t_active = srv.submit(long_job)
+    t_queued = srv.submit(gpr.job(ids[1], 32, 32, 4, gpr.lookat((0, 0, 0), (1.0, 6.0, 10.0)), fov=35, max_depth=6,
+                                  rr_start=3, seed=2))
+    srv.cancel(t_queued)
+    with pytest.raises(gpr.GprError) as e:
+        srv.wait(t_queued)
+    assert e.value.code == gpr.ERR_CANCELLED
+    time.sleep(0.05)  # the long request is on the GPU
+    srv.cancel(t_active)
+    with pytest.raises(gpr.GprError) as e:
+        srv.wait(t_active)
+    assert e.value.code == gpr.ERR_CANCELLED
+    # the slot and the frame-buffer range are free again, and a new request is unaffected
+    j = gpr.job(ids[0], 40, 24, 3, cam, fov=40, max_depth=5, rr_start=3, seed=11)
+    t = srv.submit(j)
+    got, _ = srv.wait(t)
+    want = plain(scenes[0], j)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    with pytest.raises(gpr.GprError):
+        srv.cancel(t)  # finished and collected: the ticket is gone
+    st = srv.stats()
+    assert st.cancelled == 2 and st.completed == 1
+    srv.free()
