@@ -221,6 +221,22 @@ def test_checkpoint_restore_continues_bit_exact():
     rr.free()
 
 
+def test_cli_render_checkpoint_resume(tmp_path, capsys):
+    # python -m gpr: 2 samples + checkpoint, then 1 more from the checkpoint, equals 3 at once
+    import os
+    from gpr.__main__ import main
+    scene = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cbox", "cbox.xml")
+    common = ["--depth", "4", "--rng", "pcg", "--seed", "5"]
+    assert main([scene, "--spp", "2", "--out", str(tmp_path / "a.png"), "--checkpoint", str(tmp_path / "c.npz")] + common) == 0
+    assert main([scene, "--spp", "1", "--resume", str(tmp_path / "c.npz"), "--out", str(tmp_path / "b.png")] + common) == 0
+    assert main([scene, "--spp", "3", "--out", str(tmp_path / "full.png")] + common) == 0
+    out = capsys.readouterr().out
+    assert "SpendTime" in out and "(3 samples)" in out
+    b, full = _png_rgb(str(tmp_path / "b.png")), _png_rgb(str(tmp_path / "full.png"))
+    assert b.shape == (600, 800, 3)
+    assert np.array_equal(b, full)
+
+
 def test_reference_cbox_scene_file_bit_exact():
     # The reference's own scene file (scene/CBox/cbox.xml: OBJ spheres with
     # mirror and glass, Lambert walls, area light) through the loader, at its

@@ -391,3 +391,12 @@ def test_unknown_extension(tmp_path):
     with pytest.raises(gpr.GprError) as e:
         gpr.Scene.parse(write(tmp_path, "a.ply", "ply"))
     assert e.value.code == gpr.ERR_UNSUPPORTED
+
+
+def test_cli_help_and_load_error(capsys):
+    from gpr.__main__ import main
+    with pytest.raises(SystemExit) as e:
+        main(["--help"])
+    assert e.value.code == 0
+    assert main([os.path.join(CBOX, "no_such_scene.xml"), "--spp", "1"]) == 1
+    assert "cannot open" in capsys.readouterr().err
