@@ -225,12 +225,18 @@ GPR_D int trav_step(const SceneView& sv, Stack& st, Trav& t) {
         tn[c] = tnear;
         hitc[c] = tnear <= tfar * kSlabCons && cha[c] != kEmptyChild;
     }
+    // leaf children in ascending slot order, as a loop over this lane's own
+    // leaves: a wave runs the triangle test as often as its busiest lane
+    // needs it, not once per slot that any lane has a leaf in
+    uint32_t leaves = 0;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        if (hitc[c] && cha[c] < 0) {
-            tri_rec_hit(sv.tris, sv.ntri, ~cha[c], r, t.tmax, t.hit, t.found);
-            if (ANY && t.found) return kTravDone;
-        }
+    for (int c = 0; c < 4; ++c) leaves |= (hitc[c] && cha[c] < 0) ? 1u << c : 0u;
+    while (leaves != 0) {
+        const int c = __builtin_ctz(leaves);
+        leaves &= leaves - 1;
+        const int32_t leaf = c == 0 ? cha[0] : c == 1 ? cha[1] : c == 2 ? cha[2] : cha[3];
+        tri_rec_hit(sv.tris, sv.ntri, ~leaf, r, t.tmax, t.hit, t.found);
+        if (ANY && t.found) return kTravDone;
     }
     uint32_t key[4];
 #pragma unroll
