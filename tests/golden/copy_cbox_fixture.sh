@@ -14,3 +14,6 @@ cp "$src/cbox(sphere).xml" "$dst/cbox_sphere.xml"
 for m in backwall leftwall light rightwall sphere1 sphere2 walls; do
     cp "$src/meshes/$m.obj" "$dst/meshes/$m.obj"
 done
+# The reference's published render of cbox(sphere).xml (800x600, 3006 samples,
+# images/3006spp.png): the converged-image fixture of the end-to-end test.
+cp /root/reference/images/3006spp.png "$dst/cbox_sphere_3006spp.png"

@@ -414,3 +414,15 @@ def test_texture_handle_validation():
     with pytest.raises(gpr.GprError):
         s.set_env_map(3)
     s.free()
+
+
+def test_published_cbox_render_fixture_decodes():
+    # the reference's images/3006spp.png (an 8-bit RGB PNG with row filters):
+    # 800 x 600, every value one of the 256 ReadImage levels pow(c / 255.f, 2.2f)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cbox", "cbox_sphere_3006spp.png")
+    lin = gpr.read_image(path)
+    assert lin.shape == (600, 800, 3) and lin.dtype == np.float32
+    lut = np.array([np.float32(np.float64(np.float32(c) / np.float32(255)) ** np.float64(np.float32(2.2)))
+                    for c in range(256)], np.float32)
+    assert np.isin(np.unique(lin), lut).all()
+    assert float(lin.min()) >= 0.0 and float(lin.max()) <= 1.0
