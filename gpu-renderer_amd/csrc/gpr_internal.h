@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -14,6 +16,22 @@ namespace gpr {
 // ------------------------------------------------------------ errors ---
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
+
+// Runs an entry point's body and turns a C++ exception escaping it (an
+// allocation failure while parsing a file, say) into a status code, so that
+// no exception crosses the C ABI.
+template <class F>
+int guarded(const char* what, F&& f) noexcept {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(GPR_ERR_OOM, std::string(what) + ": host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(GPR_ERR_INVALID, std::string(what) + ": " + e.what());
+    } catch (...) {
+        return fail(GPR_ERR_INVALID, std::string(what) + ": unexpected exception");
+    }
+}
 #define GPR_HIP(expr)                                                                   \
     do {                                                                                \
         hipError_t e_ = (expr);                                                         \

@@ -296,7 +296,7 @@ extern "C" int gpr_image_read(const char* path, float** rgb, uint32_t* w, uint32
     if (!path || !rgb || !w || !h) return gpr::fail(GPR_ERR_INVALID, "gpr_image_read: null argument");
     std::vector<float> v;
     uint32_t iw = 0, ih = 0;
-    GPR_TRY(gpr::read_image(path, &iw, &ih, &v));
+    GPR_TRY(gpr::guarded("gpr_image_read", [&] { return gpr::read_image(path, &iw, &ih, &v); }));
     float* p = static_cast<float*>(std::malloc(v.size() * sizeof(float)));
     if (!p) return gpr::fail(GPR_ERR_OOM, "gpr_image_read: host allocation failed");
     std::memcpy(p, v.data(), v.size() * sizeof(float));

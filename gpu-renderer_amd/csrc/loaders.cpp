@@ -51,8 +51,10 @@ struct XNode {
 };
 
 struct XmlParser {
+    static constexpr int kMaxDepth = 256;  // element nesting (parsing, the scene walk and teardown recurse per level)
     const std::string& s;
     size_t i = 0;
+    int depth = 0;
     std::string err;
     explicit XmlParser(const std::string& src) : s(src) {}
     void ws() {
@@ -94,6 +96,15 @@ struct XmlParser {
         return s.substr(b, i - b);
     }
     std::unique_ptr<XNode> element() {
+        struct Level {
+            int& d;
+            explicit Level(int& x) : d(++x) {}
+            ~Level() { --d; }
+        } level(depth);
+        if (depth > kMaxDepth) {
+            fail_at("elements nested deeper than " + std::to_string(kMaxDepth) + " levels");
+            return nullptr;
+        }
         if (!skip_misc()) return nullptr;
         if (i >= s.size() || s[i] != '<') {
             fail_at("expected an element");
@@ -870,6 +881,15 @@ int load_gprs(const std::string& path, gpr_scene* s) {
     FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) return fail(GPR_ERR_IO, "cannot open " + path);
     std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+    // every array's claimed size is checked against the bytes left in the
+    // file before anything is allocated for it
+    if (std::fseek(f, 0, SEEK_END) != 0) return fail(GPR_ERR_IO, "cannot seek " + path);
+    const long fsize = std::ftell(f);
+    if (fsize < 0 || std::fseek(f, 0, SEEK_SET) != 0) return fail(GPR_ERR_IO, "cannot seek " + path);
+    auto fits = [&](uint64_t bytes) {
+        const long at = std::ftell(f);
+        return at >= 0 && at <= fsize && bytes <= (uint64_t)(fsize - at);
+    };
     char magic[4];
     uint32_t ver = 0, nm = 0, w = 0, h = 0, has_env = 0;
     float fov = 0, c2w[16], env[3];
@@ -888,7 +908,8 @@ int load_gprs(const std::string& path, gpr_scene* s) {
             float uvo[2], uvs[2];
             if (!rd(f, &tw) || !rd(f, &th) || !rd(f, uvo, 2) || !rd(f, uvs, 2))
                 return fail(GPR_ERR_PARSE, path + ": truncated texture header");
-            if (tw == 0 || th == 0 || (uint64_t)tw * th > (1ull << 28)) return fail(GPR_ERR_PARSE, path + ": implausible texture size");
+            if (tw == 0 || th == 0 || (uint64_t)tw * th > (1ull << 28) || !fits(12ull * tw * th))
+                return fail(GPR_ERR_PARSE, path + ": implausible texture size");
             std::vector<float> px(3ull * tw * th);
             if (!rd(f, px.data(), px.size())) return fail(GPR_ERR_PARSE, path + ": truncated texels");
             GPR_TRY(gpr_scene_add_texture(s, px.data(), tw, th, uvo, uvs, &id));
@@ -902,7 +923,8 @@ int load_gprs(const std::string& path, gpr_scene* s) {
         uint32_t np, nn, nuv, nt;
         if (std::fread(&m, 1, mat_bytes, f) != mat_bytes || !rd(f, &np) || !rd(f, &nn) || !rd(f, &nuv) || !rd(f, &nt))
             return fail(GPR_ERR_PARSE, path + ": truncated mesh header");
-        if (np > (1u << 30) || nn > (1u << 30) || nuv > (1u << 30) || nt > (1u << 30))
+        if (np > (1u << 30) || nn > (1u << 30) || nuv > (1u << 30) || nt > (1u << 30) ||
+            !fits(12ull * np + 12ull * nn + 8ull * nuv + 36ull * nt))
             return fail(GPR_ERR_PARSE, path + ": implausible mesh size");
         std::vector<float> p(3ull * np), n(3ull * nn), uv(2ull * nuv);
         std::vector<int32_t> c(9ull * nt);
@@ -941,13 +963,15 @@ extern "C" int gpr_scene_load(const char* path, gpr_scene** out) {
     if (!path || !out) return fail(GPR_ERR_INVALID, "gpr_scene_load: null argument");
     gpr_scene* s = nullptr;
     GPR_TRY(gpr_scene_create(&s));
-    const std::string p(path), e = ext_of(p);
-    int rc;
-    if (e == ".xml") rc = load_xml(p, s);
-    else if (e == ".obj") rc = load_obj_scene(p, s);
-    else if (e == ".gprs") rc = load_gprs(p, s);
-    else rc = fail(GPR_ERR_UNSUPPORTED, "unknown scene format '" + e + "' (expected .xml, .obj or .gprs)");
-    if (rc == GPR_OK) rc = gpr_scene_commit(s);
+    const int rc = guarded("gpr_scene_load", [&] {
+        const std::string p(path), e = ext_of(p);
+        int r;
+        if (e == ".xml") r = load_xml(p, s);
+        else if (e == ".obj") r = load_obj_scene(p, s);
+        else if (e == ".gprs") r = load_gprs(p, s);
+        else r = fail(GPR_ERR_UNSUPPORTED, "unknown scene format '" + e + "' (expected .xml, .obj or .gprs)");
+        return r == GPR_OK ? gpr_scene_commit(s) : r;
+    });
     if (rc != GPR_OK) {
         gpr_scene_free(s);
         return rc;
@@ -962,13 +986,15 @@ extern "C" int gpr_scene_parse(const char* path, gpr_scene** out) {
     if (!path || !out) return fail(GPR_ERR_INVALID, "gpr_scene_parse: null argument");
     gpr_scene* s = nullptr;
     GPR_TRY(gpr_scene_create(&s));
-    const std::string p(path), e = ext_of(p);
-    int rc;
-    if (e == ".xml") rc = load_xml(p, s);
-    else if (e == ".obj") rc = load_obj_scene(p, s);
-    else if (e == ".gprs") rc = load_gprs(p, s);
-    else rc = fail(GPR_ERR_UNSUPPORTED, "unknown scene format '" + e + "' (expected .xml, .obj or .gprs)");
-    if (rc == GPR_OK) rc = gpr_scene_flatten(s);
+    const int rc = guarded("gpr_scene_parse", [&] {
+        const std::string p(path), e = ext_of(p);
+        int r;
+        if (e == ".xml") r = load_xml(p, s);
+        else if (e == ".obj") r = load_obj_scene(p, s);
+        else if (e == ".gprs") r = load_gprs(p, s);
+        else r = fail(GPR_ERR_UNSUPPORTED, "unknown scene format '" + e + "' (expected .xml, .obj or .gprs)");
+        return r == GPR_OK ? gpr_scene_flatten(s) : r;
+    });
     if (rc != GPR_OK) {
         gpr_scene_free(s);
         return rc;

@@ -9,6 +9,7 @@ No GPU is used: gpr_scene_parse flattens on the host only.
 """
 import math
 import os
+import struct
 
 import numpy as np
 import pytest
@@ -305,6 +306,18 @@ def test_xml_errors(tmp_path, body, code, needle):
     assert needle in str(e.value)
 
 
+def test_xml_nesting_limit(tmp_path):
+    # parsing, the scene walk and teardown recurse per level: a pathological
+    # nesting depth is a parse error, not a stack overflow
+    deep = "<hide>" * 100000 + "</hide>" * 100000
+    p = write(tmp_path, "deep.xml", SCENE_HEAD + deep + "\n</scene>")
+    with pytest.raises(gpr.GprError) as e:
+        gpr.Scene.parse(p)
+    assert e.value.code == gpr.ERR_PARSE and "nested deeper" in str(e.value)
+    ok = "<hide>" * 200 + "<shape type=\"cube\"/>" + "</hide>" * 200
+    gpr.Scene.parse(write(tmp_path, "ok.xml", SCENE_HEAD + ok + "\n<shape type=\"rectangle\"/>\n</scene>"))
+
+
 def test_hide_skips_subtree(tmp_path):
     xml = SCENE_HEAD + """
   <hide><shape type="cube"/></hide>
@@ -385,6 +398,16 @@ def test_gprs_rejects_truncated(tmp_path):
     (tmp_path / "magic.gprs").write_bytes(b"XXXX" + data[4:])
     with pytest.raises(gpr.GprError):
         gpr.Scene.parse(str(tmp_path / "magic.gprs"))
+    # a header that claims ~2^30 triangles in a small file is rejected before
+    # anything is allocated for them (first mesh header after the 208-byte v2
+    # scene header of a texture-free scene: material, np, nn, nuv, nt)
+    nt_at = 208 + gpr.MATERIAL_DTYPE.itemsize + 12
+    big = bytearray(data)
+    big[nt_at:nt_at + 4] = struct.pack("<I", (1 << 30) - 1)
+    (tmp_path / "big.gprs").write_bytes(bytes(big))
+    with pytest.raises(gpr.GprError) as e:
+        gpr.Scene.parse(str(tmp_path / "big.gprs"))
+    assert e.value.code == gpr.ERR_PARSE and "implausible mesh size" in str(e.value)
 
 
 def test_unknown_extension(tmp_path):
