@@ -287,6 +287,8 @@ extern "C" {
 const char* gpr_last_error(void) { return t_err.c_str(); }
 
 int gpr_set_device(int device) {
+    // per-device state (traversal resources, sort scratch) is kept in tables of 16
+    if (device < 0 || device >= 16) return fail(GPR_ERR_INVALID, "gpr_set_device: device index outside 0..15");
     GPR_HIP(hipSetDevice(device));
     g_device = device;
     return GPR_OK;

@@ -107,7 +107,8 @@ typedef struct gpr_renderer gpr_renderer;
 
 /* ------------------------------------------------------------ runtime --- */
 const char* gpr_last_error(void);
-/* Select the HIP device used by handles created afterwards (default 0). */
+/* Select the HIP device used by handles created afterwards (default 0;
+ * devices 0..15, GPR_ERR_INVALID beyond). */
 int gpr_set_device(int device);
 /* Stream (hipStream_t) for all subsequent work; NULL = null stream. */
 int gpr_set_stream(void* hip_stream);

@@ -57,3 +57,10 @@ def test_mesh_validation_errors_without_gpu():
         assert e.code == gpr.ERR_INVALID
     s.add_mesh(p)  # valid
     assert gpr.lib.gpr_last_error() is not None
+
+
+def test_set_device_rejects_indices_outside_the_tables():
+    # checked before any HIP call, so this runs without a GPU
+    for bad in (-1, 16, 1 << 20):
+        assert gpr.lib.gpr_set_device(bad) == gpr.ERR_INVALID
+        assert b"0..15" in gpr.lib.gpr_last_error()
