@@ -237,16 +237,19 @@ def test_cli_render_checkpoint_resume(tmp_path, capsys):
     assert np.array_equal(b, full)
 
 
-def test_reference_cbox_scene_file_bit_exact():
-    # The reference's own scene file (scene/CBox/cbox.xml: OBJ spheres with
-    # mirror and glass, Lambert walls, area light) through the loader, at its
-    # own 800x600 film, against the oracle on the same flattened scene.
+@pytest.mark.parametrize("name,rng_mode", [("cbox.xml", gpr.RNG_PCG), ("cbox_sphere.xml", gpr.RNG_PCG),
+                                           ("cbox_sphere.xml", gpr.RNG_REFERENCE), ("cbox_mesh.xml", gpr.RNG_PCG)])
+def test_reference_cbox_scene_file_bit_exact(name, rng_mode):
+    # The reference's own scene files (scene/CBox/cbox.xml: OBJ spheres with
+    # mirror and glass; cbox(sphere).xml: the same as tessellated sphere
+    # shapes; cbox(mesh).xml: OBJ walls) through the loader, at their own
+    # 800x600 film, against the oracle on the same flattened scene.
     import os
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cbox", "cbox.xml")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cbox", name)
     s = gpr.Scene.load(path)
-    print("[render] cbox.xml 800x600 1 spp depth 5 pcg ref-order", flush=True)
-    g, cnt, _, _ = gpu_render(s, 1, 5, gpr.RNG_PCG, 5, gpr.TRAVERSAL_REF)
-    o, ocnt = oracle_render(s, 1, 5, gpr.RNG_PCG, 5)
+    print(f"[render] {name} 800x600 1 spp depth 5 rng {rng_mode} ref-order", flush=True)
+    g, cnt, _, _ = gpu_render(s, 1, 5, rng_mode, 5, gpr.TRAVERSAL_REF)
+    o, ocnt = oracle_render(s, 1, 5, rng_mode, 5)
     assert np.array_equal(cnt, ocnt)
     bad = np.nonzero(np.any(g.view(np.uint32) != o.view(np.uint32), axis=1))[0]
     assert len(bad) == 0, (len(bad), bad[:10])
