@@ -1008,12 +1008,14 @@ RenderArgs make_args(gpr_renderer* r) {
     return a;
 }
 
+constexpr int kRenderNT = 256, kRenderSL = 8;  // block size and LDS stack entries per lane of the single-scene renderer's traversal
+
 // One sample: gen -> extend(primary) -> max_depth x {shade, shadow, extend} -> finish.
 int launch_sample(gpr_renderer* r, hipStream_t st) {
     const RenderArgs a = make_args(r);
     const bool ref = r->opts.traversal == GPR_TRAVERSAL_REF;
-    const dim3 tg(r->grid_trace), tb(kTraceBlock);
-    constexpr int NT = kTraceBlock, SL = kStackLds;
+    const dim3 tg(r->grid_trace), tb(kRenderNT);
+    constexpr int NT = kRenderNT, SL = kRenderSL;
     hipLaunchKernelGGL(k_gen<false>, dim3(div_up(r->npix, 256)), dim3(256), 0, st, a);
     if (ref) hipLaunchKernelGGL((k_extend<true, true, false, NT, SL>), tg, tb, 0, st, a, 0u);
     else hipLaunchKernelGGL((k_extend<true, false, false, NT, SL>), tg, tb, 0, st, a, 0u);
@@ -1087,12 +1089,12 @@ int renderer_alloc(gpr_renderer* r) {
         GPR_TRY(alloc_queue(r->qmem[k], r->q[k], r->npix, k < 2 ? &r->seeds[k] : nullptr, false));
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend<false, false, false, kTraceBlock, kStackLds>,
-                                                     kTraceBlock, 0) != hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend<false, false, false, kRenderNT, kRenderSL>,
+                                                     kRenderNT, 0) != hipSuccess)
         return fail(GPR_ERR_HIP, "occupancy query failed");
     r->grid_trace = std::max(1, cus * std::max(1, per_cu));
-    r->grid_trace = std::min<int>(r->grid_trace, (int)div_up(npix, kTraceBlock));
-    GPR_TRY(r->spill.alloc((size_t)r->grid_trace * kTraceBlock * kStackGlobal * sizeof(int2)));
+    r->grid_trace = std::min<int>(r->grid_trace, (int)div_up(npix, kRenderNT));
+    GPR_TRY(r->spill.alloc((size_t)r->grid_trace * kRenderNT * kStackGlobal * sizeof(int2)));
     return GPR_OK;
 }
 }  // namespace
