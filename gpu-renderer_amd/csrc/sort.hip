@@ -36,11 +36,12 @@ constexpr int kQuarters = 4;
 constexpr int kTile = 16384;        // keys per histogram tile
 constexpr int kSubTile = kTile / kQuarters;  // keys per scatter tile
 constexpr int kTileThreads = 1024;
-constexpr int kLocalThreads = 256;
+constexpr int kLocalThreads = 512;
 constexpr int kFastCap = 1536;      // bucket size of the LDS fast path (A and B)
 constexpr int kSub = 1024;          // sub-buckets of the LDS MSD split
 constexpr int kSubMax = 48;         // largest sub-bucket sorted by one thread
 constexpr uint32_t kWin = 8;        // sub-bucket entries a rank reads at once
+static_assert(kSub % kLocalThreads == 0, "sub-bucket scan: whole counts per thread");
 
 // Top-digit width: ~1K keys per bucket, between 1 and 11 bits.
 uint32_t digit_bits(uint32_t n) {
@@ -303,19 +304,20 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const u
             r[j] = i < n ? atomicAdd(&cnt[sb[j]], 1u) : 0u;
         }
         __syncthreads();
-        uint32_t c4[4], s4 = 0;
+        constexpr int kQ = kSub / kLocalThreads;  // sub-bucket counts scanned per thread
+        uint32_t cq[kQ], sq = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            c4[k] = cnt[4 * threadIdx.x + k];
-            s4 += c4[k];
-            if (c4[k] > kSubMax) crowded = 1;
+        for (int k = 0; k < kQ; ++k) {
+            cq[k] = cnt[kQ * threadIdx.x + k];
+            sq += cq[k];
+            if (cq[k] > kSubMax) crowded = 1;
         }
         uint32_t tot;
-        uint32_t run = scan_incl<kLocalThreads>(s4, sh, &tot) - s4;
+        uint32_t run = scan_incl<kLocalThreads>(sq, sh, &tot) - sq;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            start[4 * threadIdx.x + k] = run;
-            run += c4[k];
+        for (int k = 0; k < kQ; ++k) {
+            start[kQ * threadIdx.x + k] = run;
+            run += cq[k];
         }
         __syncthreads();
         if (!crowded) {
