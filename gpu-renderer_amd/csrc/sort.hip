@@ -40,7 +40,7 @@ constexpr int kLocalThreads = 512;
 constexpr int kFastCap = 1536;      // bucket size of the LDS fast path (A and B)
 constexpr int kSub = 1024;          // sub-buckets of the LDS MSD split
 constexpr int kSubMax = 48;         // largest sub-bucket sorted by one thread
-constexpr uint32_t kWin = 8;        // sub-bucket entries a rank reads at once
+constexpr uint32_t kWin = 4;        // sub-bucket entries a rank reads at once
 static_assert(kSub % kLocalThreads == 0, "sub-bucket scan: whole counts per thread");
 
 // Top-digit width: ~1K keys per bucket, between 1 and 11 bits.
