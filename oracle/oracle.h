@@ -103,6 +103,10 @@ uint32_t orc_left_shift3(uint32_t x);
 uint32_t orc_encode_morton3(float x, float y, float z);
 void orc_centroid_bounds(const float* pos, uint32_t ntri, float bmin[3], float bmax[3]);
 void orc_morton_keys(const float* pos, uint32_t ntri, uint32_t* keys);
+/* (key, index) pairs sorted by key, equal keys in index order.  Parity
+ * unpinned for ties: the reference's std::sort (bvh.h:207-208) is not stable
+ * and its order among equal keys is the standard library's own; the stable
+ * order here is a definition, checked only as sorted + permutation. */
 void orc_sort_pairs(const uint32_t* keys, uint32_t n, uint32_t* skeys, uint32_t* sidx);
 /* Full reference build; returns number of nodes written (2n-1) or 0 for n==0. */
 uint32_t orc_lbvh_build(const float* pos, uint32_t ntri, orc_node* nodes,
