@@ -18,9 +18,10 @@
 namespace gpr {
 
 namespace {
+// Last error, stream and device are per host thread, like HIP's current device.
 thread_local std::string t_err;
-hipStream_t g_stream = nullptr;
-int g_device = 0;
+thread_local hipStream_t g_stream = nullptr;
+thread_local int g_device = 0;
 }  // namespace
 
 void set_error(const std::string& m) { t_err = m; }

@@ -107,10 +107,12 @@ typedef struct gpr_renderer gpr_renderer;
 
 /* ------------------------------------------------------------ runtime --- */
 const char* gpr_last_error(void);
-/* Select the HIP device used by handles created afterwards (default 0;
- * devices 0..15, GPR_ERR_INVALID beyond). */
+/* Select the HIP device used by handles created afterwards on the calling
+ * host thread (default 0; devices 0..15, GPR_ERR_INVALID beyond).  Like
+ * hipSetDevice, the setting is per host thread. */
 int gpr_set_device(int device);
-/* Stream (hipStream_t) for all subsequent work; NULL = null stream. */
+/* Stream (hipStream_t) for all subsequent work issued from the calling host
+ * thread; NULL = null stream.  Per host thread. */
 int gpr_set_stream(void* hip_stream);
 int gpr_synchronize(void);
 
