@@ -526,6 +526,9 @@ int gpr_scene_commit(gpr_scene* s) {
 }
 
 int gpr_scene_free(gpr_scene* s) {
+    if (s && s->renderers.load() != 0)
+        return fail(GPR_ERR_INVALID, "gpr_scene_free: " + std::to_string(s->renderers.load()) +
+                                         " renderer(s) still use this scene; free them first");
     if (s) {
         (void)hipStreamSynchronize(stream());
         delete s;

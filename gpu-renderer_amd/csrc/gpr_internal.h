@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -160,6 +161,7 @@ struct Mesh {
 
 struct gpr_scene {
     int device = 0;
+    std::atomic<uint32_t> renderers{0};  // live gpr_renderer handles on this scene: gpr_scene_free refuses while > 0
     std::vector<gpr::Mesh> meshes;
     float c2w[16];
     float fov = 40.f;

@@ -1117,6 +1117,7 @@ int gpr_renderer_create(gpr_scene* s, const gpr_render_opts* o, gpr_renderer** o
         delete r;
         return rc;
     }
+    ++s->renderers;
     *out = r;
     return GPR_OK;
 }
@@ -1263,6 +1264,7 @@ int gpr_renderer_queue_sizes(gpr_renderer* r, uint32_t* shadow, uint32_t* scatte
 int gpr_renderer_free(gpr_renderer* r) {
     if (r) {
         (void)hipStreamSynchronize(stream());
+        --r->scene->renderers;
         delete r;
     }
     return GPR_OK;

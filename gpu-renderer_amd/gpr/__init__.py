@@ -462,7 +462,7 @@ class Scene:
 
     def free(self):
         if self.h:
-            lib.gpr_scene_free(self.h)
+            check(lib.gpr_scene_free(self.h))  # refused while renderers still use the scene
             self.h = None
 
     def __del__(self):

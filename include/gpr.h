@@ -173,6 +173,8 @@ int gpr_scene_commit(gpr_scene* s);
 /* Host half of gpr_scene_commit: flatten the meshes into per-triangle arrays
  * and collect the emissive triangles (scene.cpp:7-20), without device work. */
 int gpr_scene_flatten(gpr_scene* s);
+/* Releases the scene; GPR_ERR_INVALID (nothing freed) while renderers created
+ * on it are still alive. */
 int gpr_scene_free(gpr_scene* s);
 
 typedef struct {

@@ -254,3 +254,17 @@ def test_reference_cbox_scene_file_bit_exact(name, rng_mode):
     bad = np.nonzero(np.any(g.view(np.uint32) != o.view(np.uint32), axis=1))[0]
     assert len(bad) == 0, (len(bad), bad[:10])
     assert g.sum() > 0
+
+
+def test_scene_free_refused_while_renderers_live():
+    s = gpr.Scene.cornell(16, 16)
+    r1 = gpr.Renderer(s, max_depth=2)
+    r2 = gpr.Renderer(s, max_depth=2)
+    for r in (r1, r2):
+        with pytest.raises(gpr.GprError) as e:
+            s.free()
+        assert e.value.code == gpr.ERR_INVALID and "renderer" in str(e.value)
+        r.render(1)  # the scene is intact
+        r.free()
+    s.free()
+    assert s.h is None
