@@ -605,7 +605,12 @@ struct Loader {
             int rc = load_obj(f, &m);
             if (rc != GPR_OK) return err = rc;
         } else if (type == "sphere") {
-            make_sphere(m, L.getf("radius", 1.f), L.geti("subdiv", 30));
+            // 2 .. 4096 latitude bands (4096: 33.5 M triangles); outside that the
+            // tessellation has no valid mesh or its int indices would overflow
+            const int subdiv = L.geti("subdiv", 30);
+            if (subdiv < 2 || subdiv > 4096)
+                return bad(GPR_ERR_UNSUPPORTED, "sphere subdiv " + std::to_string(subdiv) + " outside 2..4096");
+            make_sphere(m, L.getf("radius", 1.f), subdiv);
         } else if (type == "rectangle") {
             make_rectangle(m);
         } else if (type == "cube") {

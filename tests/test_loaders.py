@@ -295,6 +295,8 @@ def test_emitter_applies_to_next_shape_only(tmp_path):
      "<string name=\"filename\" value=\"missing.png\"/></texture></bsdf>", gpr.ERR_IO, "missing.png"),
     ("<shape type=\"obj\"><string name=\"filename\" value=\"missing.obj\"/></shape>", gpr.ERR_IO, "missing.obj"),
     ("<emitter type=\"envmap\"/>", gpr.ERR_PARSE, "filename"),
+    ("<shape type=\"sphere\"><integer name=\"subdiv\" value=\"100000\"/></shape>", gpr.ERR_UNSUPPORTED, "subdiv"),
+    ("<shape type=\"sphere\"><integer name=\"subdiv\" value=\"1\"/></shape>", gpr.ERR_UNSUPPORTED, "subdiv"),
     ("<emitter type=\"envmap\"><string name=\"filename\" value=\"bad.xml\"/></emitter>", gpr.ERR_UNSUPPORTED,
      "not recognised"),
 ])
