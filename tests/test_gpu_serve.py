@@ -187,3 +187,20 @@ def test_server_cancel_queued_active_and_finished(scenes):
     st = srv.stats()
     assert st.cancelled == 2 and st.completed == 1
     srv.free()
+
+
+def test_batch_keeps_its_own_copy_of_added_scenes(scenes):
+    # add_scene copies the scene into the arena: the caller may free its scene
+    # straight away and the batch still renders it exactly
+    tmp = gpr.Scene.cornell(32, 32)
+    arena = gpr.arena_bytes(tmp)
+    b = gpr.Batch(arena, capacity=4096, max_jobs=4)
+    sid = b.add_scene(tmp)
+    tmp.free()
+    j = jobs_for([sid, sid])[0]
+    b.add_job(j)
+    b.run()
+    got = b.read(0)
+    want = plain(scenes[0], j)  # the same procedural Cornell box, built again
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    b.free()
