@@ -6,6 +6,7 @@
 // the device (not to the batch) and the per-lane overflow stacks are indexed
 // by the persistent thread id.
 #include <algorithm>
+#include <type_traits>
 
 #include "traverse.h"
 
@@ -61,8 +62,12 @@ __global__ __launch_bounds__(NT) void k_trace_fast(SceneView sv, const gpr_ray* 
 // Per-lane ray refill: the lanes of a wave step their rays one node at a
 // time and a lane whose ray is done takes the next one from a wave-local pool
 // of `chunk` ray indices (one global atomic per chunk), so no lane idles while
-// the slowest ray of the wave finishes.  Every ray takes the same node steps
-// as in k_trace_fast, so the hits are identical.
+// the slowest ray of the wave finishes.  A closest-hit ray takes the same
+// node steps as in k_trace_fast, so the hits are identical.  Any hit parks
+// leaf children and tests them in wave-wide leaf rounds (TravD): the
+// occlusion answer is the same, the occluder reported may differ.
+constexpr int kLeafRound = 32;  // parked lanes of a wave that start an any-hit leaf round
+
 template <bool ANY, int NT, int SL>
 __global__ __launch_bounds__(NT) void k_trace_refill(SceneView sv, const gpr_ray* __restrict__ rays,
                                                      gpr_hit* __restrict__ hits, uint64_t n,
@@ -78,7 +83,8 @@ __global__ __launch_bounds__(NT) void k_trace_refill(SceneView sv, const gpr_ray
     bool drained = false;     // the global cursor has passed n (wave-uniform)
     bool active = false;
     uint64_t ri = 0;
-    Trav t;
+    // any hit walks with postponed leaves (leaf rounds once kLeafRound lanes have parked some)
+    std::conditional_t<ANY, TravD, Trav> t;
     while (true) {
         const unsigned long long need = __ballot(!active);
         if (need != 0ull && !(drained && pb == pe)) {
@@ -101,7 +107,10 @@ __global__ __launch_bounds__(NT) void k_trace_refill(SceneView sv, const gpr_ray
                 if (from_old || idx < ne) {
                     ri = idx;
                     const Ray3 r = load_ray(rays, ri);
-                    if (trav_begin<ANY>(sv, r, st, t)) store_hit(hits, ri, t.hit);
+                    bool done;
+                    if constexpr (ANY) done = travd_begin<ANY>(sv, r, st, t);
+                    else done = trav_begin<ANY>(sv, r, st, t);
+                    if (done) store_hit(hits, ri, t.hit);
                     else active = true;
                 }
             }
@@ -117,12 +126,31 @@ __global__ __launch_bounds__(NT) void k_trace_refill(SceneView sv, const gpr_ray
             if (drained && pb == pe) break;
             continue;
         }
-        if (active) {
-            const int rc = trav_step<ANY>(sv, st, t);
-            if (rc != kTravMore) {
-                if (rc == kTravOverflow) atomicAdd(cursor + 1, 1ull);  // stack overflow count
-                store_hit(hits, ri, t.hit);
-                active = false;
+        if constexpr (ANY) {
+            if (active && t.pmask == 0) {
+                const int rc = travd_step<ANY>(sv, st, t);
+                if (rc != kTravMore) {
+                    if (rc == kTravOverflow) atomicAdd(cursor + 1, 1ull);  // stack overflow count
+                    store_hit(hits, ri, t.hit);
+                    active = false;
+                }
+            }
+            // leaf round: when enough lanes have parked leaves, or no lane can step
+            const unsigned long long parked = __ballot(active && t.pmask != 0);
+            if (parked != 0ull && (__popcll(parked) >= kLeafRound || __ballot(active && t.pmask == 0) == 0ull)) {
+                if (active && t.pmask != 0 && travd_leaves<ANY>(sv, t)) {
+                    store_hit(hits, ri, t.hit);
+                    active = false;
+                }
+            }
+        } else {
+            if (active) {
+                const int rc = trav_step<ANY>(sv, st, t);
+                if (rc != kTravMore) {
+                    if (rc == kTravOverflow) atomicAdd(cursor + 1, 1ull);  // stack overflow count
+                    store_hit(hits, ri, t.hit);
+                    active = false;
+                }
             }
         }
     }

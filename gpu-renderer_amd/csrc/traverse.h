@@ -276,6 +276,115 @@ GPR_D int trav_step(const SceneView& sv, Stack& st, Trav& t) {
     return kTravDone;
 }
 
+// Postponed-leaf form of the BVH4 walk, used by the persistent any-hit
+// kernel.  A node step does not test its leaf children: it parks them (slot
+// mask + references) and the lane pauses; the wave tests the parked leaves of
+// its lanes together in one leaf round, so the (double precision) triangle
+// test runs with many lanes active instead of the few whose node held
+// leaves.  A lane descends before its leaves are tested; pops wait for the
+// round.  For an any-hit query the answer (occluded or not) is unchanged; the
+// occluder reported may be a different triangle than in the immediate order.
+struct TravD : Trav {
+    uint32_t pmask;    // parked leaf slots of the last visited node
+    int32_t pleaf[4];  // that node's child references
+};
+
+template <bool ANY, class Stack>
+GPR_D bool travd_begin(const SceneView& sv, const Ray3& r, Stack& st, TravD& t) {
+    t.pmask = 0;
+    return trav_begin<ANY>(sv, r, st, t);
+}
+
+// One node; pops first when the lane has no node to visit (kTravDone when the
+// stack is empty).  Leaves hit are parked in t.pmask / t.pleaf.
+template <bool ANY, class Stack>
+GPR_D int travd_step(const SceneView& sv, Stack& st, TravD& t) {
+    if (t.node < 0) {
+        bool got = false;
+        while (st.sp > 0) {
+            const int2 e = st.pop();
+            if (ANY || __int_as_float(e.y) <= t.tmax * kSlabCons) {
+                t.node = e.x;
+                got = true;
+                break;
+            }
+        }
+        if (!got) return kTravDone;
+    }
+    if ((uint32_t)t.node >= sv.nbvh4 || t.fetched > sv.nbvh4) return kTravOverflow;
+    const Ray3& r = t.r;
+    const float4* np = reinterpret_cast<const float4*>(sv.bvh4 + t.node);
+    const float4 nx = np[0 + t.sx], fx = np[1 - t.sx], ny = np[2 + t.sy], fy = np[3 - t.sy];
+    const float4 nz = np[4 + t.sz], fz = np[5 - t.sz];
+    const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+    const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+    const int32_t cha[4] = {ch.x, ch.y, ch.z, ch.w};
+    float tn[4];
+    bool hitc[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float x0 = (nxa[c] - r.o.x) * t.inv.x, x1 = (fxa[c] - r.o.x) * t.inv.x;
+        const float y0 = (nya[c] - r.o.y) * t.inv.y, y1 = (fya[c] - r.o.y) * t.inv.y;
+        const float z0 = (nza[c] - r.o.z) * t.inv.z, z1 = (fza[c] - r.o.z) * t.inv.z;
+        const float tnear = fmaxf(fmaxf(x0, y0), fmaxf(z0, r.tmin));
+        const float tfar = fminf(fminf(x1, y1), fminf(z1, t.tmax));
+        tn[c] = tnear;
+        hitc[c] = tnear <= tfar * kSlabCons && cha[c] != kEmptyChild;
+    }
+    uint32_t leaves = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        leaves |= (hitc[c] && cha[c] < 0) ? 1u << c : 0u;
+        t.pleaf[c] = cha[c];
+    }
+    t.pmask = leaves;
+    uint32_t key[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const bool go = hitc[c] && cha[c] >= 0 && tn[c] <= t.tmax * kSlabCons;
+        key[c] = go ? ((__float_as_uint(fmaxf(tn[c], 0.f)) & ~3u) | (uint32_t)c) : 0xffffffffu;
+    }
+#define GPR_CSWAP(a, b)                           \
+    {                                             \
+        const uint32_t lo_ = min(key[a], key[b]); \
+        const uint32_t hi_ = max(key[a], key[b]); \
+        key[a] = lo_;                             \
+        key[b] = hi_;                             \
+    }
+    GPR_CSWAP(0, 1) GPR_CSWAP(2, 3) GPR_CSWAP(0, 2) GPR_CSWAP(1, 3) GPR_CSWAP(1, 2)
+#undef GPR_CSWAP
+    const int cnt = (key[0] != 0xffffffffu) + (key[1] != 0xffffffffu) + (key[2] != 0xffffffffu) +
+                    (key[3] != 0xffffffffu);
+    ++t.fetched;
+    if (cnt > 0) {
+        for (int k = cnt - 1; k >= 1; --k) {
+            const int c = key[k] & 3;
+            if (!st.push(make_int2(cha[c], __float_as_int(tn[c])))) return kTravOverflow;
+        }
+        t.node = cha[key[0] & 3];
+    } else {
+        t.node = -1;
+    }
+    return kTravMore;
+}
+
+// Tests the parked leaves in slot order; true when an any-hit query is done.
+template <bool ANY>
+GPR_D bool travd_leaves(const SceneView& sv, TravD& t) {
+    uint32_t m = t.pmask;
+    t.pmask = 0;
+    while (m != 0) {
+        const int c = __builtin_ctz(m);
+        m &= m - 1;
+        const int32_t leaf = c == 0 ? t.pleaf[0] : c == 1 ? t.pleaf[1] : c == 2 ? t.pleaf[2] : t.pleaf[3];
+        tri_rec_hit(sv.tris, sv.ntri, ~leaf, t.r, t.tmax, t.hit, t.found);
+        if (ANY && t.found) return true;
+    }
+    return false;
+}
+
 // Closest hit (ANY=false) or any hit (ANY=true) against the BVH4.  Returns
 // false only if the stack capacity was exceeded (the ray is then reported
 // with the hits found so far and the caller raises an error flag).
