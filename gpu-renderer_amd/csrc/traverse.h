@@ -7,9 +7,11 @@
 //              reference semantics and for the traversal statistics.
 //   * fast_* : packed 128-B BVH4 nodes (one cache line per node), a robust slab
 //              test on all four children, nearest-first descent, leaf triangles
-//              tested as soon as their parent is visited, and a short per-lane
-//              stack in LDS that spills its oldest entries to a global overflow
-//              area.  Closest-hit results equal the reference's up to ties of t.
+//              tested as soon as their parent is visited (the persistent any-hit
+//              kernel instead parks them for wave-wide leaf rounds, TravD), and a
+//              short per-lane stack in LDS that spills its oldest entries to a
+//              global overflow area.  Closest-hit results equal the reference's up
+//              to ties of t.
 #pragma once
 
 #include "dev_math.h"
