@@ -22,6 +22,38 @@ def identical(n):
     return np.tile(t, (n, 1))
 
 
+def planar_grid(m, z=0.5, seed=3):
+    """2 m^2 triangles tiling [0,1]^2 in the plane z = const, interior vertices
+    jittered by under half a cell (no overlaps, so no exact-t ties): the
+    centroid bounds have zero z extent (0/0 in the key quantizer) and every
+    BVH box is flat in z."""
+    rng = np.random.default_rng(seed)
+    g = np.linspace(0, 1, m + 1, dtype=np.float32)
+    x, y = np.meshgrid(g, g, indexing="ij")
+    j = rng.uniform(-0.3 / m, 0.3 / m, (2, m + 1, m + 1)).astype(np.float32)
+    j[:, [0, -1], :] = 0
+    j[:, :, [0, -1]] = 0
+    v = np.stack([x + j[0], y + j[1], np.full_like(x, z)], -1)
+    a, b, c, d = v[:-1, :-1], v[1:, :-1], v[1:, 1:], v[:-1, 1:]
+    t0 = np.concatenate([a, b, c], -1).reshape(-1, 9)
+    t1 = np.concatenate([a, c, d], -1).reshape(-1, 9)
+    return np.concatenate([t0, t1]).astype(np.float32)
+
+
+def collinear(n, seed=4):
+    """Tilted triangles whose box centres all lie on the line y = z = 0.5
+    (two flat axes of the centroid bounds); all coordinates but x are exact
+    binary fractions, so the centres are exact."""
+    rng = np.random.default_rng(seed)
+    x0 = rng.random(n).astype(np.float32)
+    w = np.float32(0.01)
+    t = np.zeros((n, 3, 3), np.float32)
+    t[:, 0] = np.stack([x0, np.full(n, 0.375), np.full(n, 0.5)], -1)
+    t[:, 1] = np.stack([x0 + w, np.full(n, 0.625), np.full(n, 0.375)], -1)
+    t[:, 2] = np.stack([x0 + w / 2, np.full(n, 0.5), np.full(n, 0.625)], -1)
+    return t.reshape(n, 9)
+
+
 def scene_from_soup(pos):
     return gpr.Scene.soup(pos).commit()
 

@@ -18,6 +18,8 @@ CASES = {
     "dups": lambda: G.duplicate_heavy(20000),
     "identical": lambda: G.identical(1000),
     "identical_big": lambda: G.identical(9000),  # one bucket larger than the LDS tile
+    "planar": lambda: G.planar_grid(90),  # zero z extent of the centroid bounds
+    "collinear": lambda: G.collinear(5000),  # zero y and z extent
 }
 
 

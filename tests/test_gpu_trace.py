@@ -36,16 +36,27 @@ def compare_fast(h, ref, pos, rays, label):
 
 @pytest.mark.parametrize("n", [1, 3, 4096, 100000])
 def test_closest_and_any(n):
-    pos = G.soup(n)
+    check_closest_and_any(G.soup(n), f"n={n}")
+
+
+@pytest.mark.parametrize("name", ["planar", "collinear"])
+def test_closest_and_any_flat_geometry(name):
+    # every box of the planar grid is flat in z (zero-thickness slabs); the
+    # collinear scene's centroid bounds are flat in y and z
+    pos = G.planar_grid(90) if name == "planar" else G.collinear(5000)
+    check_closest_and_any(pos, name)
+
+
+def check_closest_and_any(pos, label):
     s = G.scene_from_soup(pos)
     onodes, _, _ = O.lbvh(pos)
     rays = rays_for(20000)
-    print(f"[trace] n={n}", flush=True)
+    print(f"[trace] {label}", flush=True)
     ref, _, _ = O.trace(onodes, pos, rays)
     href = s.trace(rays, ref_order=True)
     assert np.array_equal(href.view(np.uint32), ref.view(np.uint32))  # bit-exact, misses included
     hfast = s.trace(rays)
-    compare_fast(hfast, ref, pos, rays, f"closest n={n}")
+    compare_fast(hfast, ref, pos, rays, f"closest {label}")
     sh = gpr.gen_shadow_rays(2, 2, rays)
     aref, _, _ = O.trace(onodes, pos, sh, any_hit=True)
     a_ref_gpu = s.trace(sh, any_hit=True, ref_order=True)

@@ -70,6 +70,24 @@ def test_morton_keys_quantize_against_bounds():
         assert k == interleave_loop(x, y, z)
 
 
+@pytest.mark.parametrize("flat", [(2,), (1, 2), (0, 1, 2)])
+def test_morton_keys_flat_axes_quantize_to_zero(flat):
+    # a flat axis of the centroid bounds makes (c - min) / (max - min) 0/0 = NaN:
+    # its coordinate is defined as 0, the other axes quantize as usual
+    rng = np.random.default_rng(len(flat))
+    cells = [tuple(int(v) for v in rng.integers(0, 1024, 3)) for _ in range(300)]
+    pos = grid_soup(cells).reshape(-1, 3, 3)
+    for a in flat:
+        pos[:, :, a] = 0.25
+    keys = O.morton_keys(pos.reshape(-1, 9))
+    for (x, y, z), k in zip([(0, 0, 0), (1023, 1023, 1023)] + cells, keys):
+        q = [0 if a in flat else v for a, v in enumerate((x, y, z))]
+        assert k == interleave_loop(*q)
+    nodes, sk, si = O.lbvh(pos.reshape(-1, 9))
+    assert len(nodes) == 2 * len(keys) - 1
+    assert np.array_equal(sk, np.sort(keys)) and np.array_equal(keys[si], sk)
+
+
 def test_morton_keys_use_bounding_box_centroid():
     # a triangle whose vertex mean differs from its box centre
     tri = np.array([[0, 0, 0, 1, 0, 0, 1, 1, 0]], np.float32)
