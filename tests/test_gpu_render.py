@@ -93,6 +93,37 @@ def test_sphere_grid_fast_traversal():
     assert frac >= 0.995
 
 
+def test_sphere_grid_full_size_one_sample():
+    # the full benchmark scene and film (8x8 UV spheres of 64x32 segments on a
+    # ground plane, 253,958 triangles, 1920x1080, depth 8, PCG seed 7), one
+    # sample: reference-order traversal bit for bit, the fast BVH4 traversal
+    # per pixel within 1e-4 of the oracle
+    import os
+    import time
+    s = gpr.Scene.sphere_grid(1920, 1080, 32)
+    f = s.flat()
+    info = f["info"]
+    assert len(f["pos"]) == 253958
+    osc = O.make_scene(f, np.array(info.c2w[:], np.float32), info.fov_deg, info.width, info.height,
+                       env=(list(info.env) if info.has_env else None), info=info)
+    nodes, _, _ = O.lbvh(f["pos"])
+    t0 = time.time()
+    o, ocnt = O.render(osc, nodes, 1, 8, gpr.RNG_PCG, seed=7, rr_start=3,
+                       nthreads=min(16, os.cpu_count() or 1))
+    lit = float((o.sum(axis=1) > 0).mean())
+    print(f"[render] full-size oracle sample {time.time() - t0:.1f} s, lit pixels {lit:.3f}", flush=True)
+    assert np.array_equal(ocnt, np.ones_like(ocnt)) and lit > 0.1
+    g, cnt, _, _ = gpu_render(s, 1, 8, gpr.RNG_PCG, 7, gpr.TRAVERSAL_REF)
+    assert np.array_equal(cnt, ocnt)
+    bad = np.nonzero(np.any(g.view(np.uint32) != o.view(np.uint32), axis=1))[0]
+    assert len(bad) == 0, (len(bad), bad[:10], g[bad[:3]], o[bad[:3]])
+    gf, _, _, _ = gpu_render(s, 1, 8, gpr.RNG_PCG, 7, gpr.TRAVERSAL_FAST)
+    frac = rel_close(gf, o).mean()
+    print(f"[render] full-size fast traversal agreement {frac:.6f}", flush=True)
+    assert frac >= 0.995
+    assert abs(float(gf.mean()) - float(o.mean())) <= 1e-3 * float(o.mean())
+
+
 def test_env_light_and_reset():
     s = gpr.Scene.cornell(16, 16)
     r = gpr.Renderer(s, max_depth=3, rng_mode=gpr.RNG_PCG, seed=3)
