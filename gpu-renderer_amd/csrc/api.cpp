@@ -33,6 +33,7 @@ hipStream_t stream() { return g_stream; }
 int device_id() { return g_device; }
 
 int trace_overflow_count(unsigned long long* out);
+int trace_overflow_accumulate(unsigned long long* acc, hipStream_t st);
 
 // ------------------------------------------------------ host transforms --
 // Matrix4x4 / Transform semantics of src/renderer/transform.{h,cpp}: float
@@ -627,6 +628,73 @@ int gpr_sort_pairs_device(const uint32_t* keys, uint32_t* sk, uint32_t* si, uint
     return sort_pairs(keys, sk, si, n, b.p, b.bytes, &calls[g_device & 15], stream());
 }
 
+namespace {
+// Host-buffer traces above this many rays move in chunks, so that one chunk's
+// copies overlap another's traversal.
+constexpr uint64_t kHostChunk = 1ull << 20;
+
+// Rays go in on one copy stream and hits come out on another; the chunks'
+// traversals run in order on the caller's stream, each waiting for its rays
+// and each copy-out waiting for its traversal.  The copy-out of chunk c-1 is
+// issued after chunk c's copy-in and launch: a copy to or from pageable host
+// memory can block the calling thread until it completes, and this order keeps
+// the device busy meanwhile.  Results are those of one launch over the batch.
+int trace_host_pipelined(gpr_scene* s, const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64_t n,
+                         uint32_t flags, gpr_ray* dr, gpr_hit* dh) {
+    hipStream_t st = stream();
+    const uint64_t nch = (n + kHostChunk - 1) / kHostChunk;
+    DevBuf dov;
+    GPR_TRY(dov.alloc(sizeof(unsigned long long)));
+    hipStream_t cin = nullptr, cout = nullptr;
+    std::vector<hipEvent_t> ev;
+    auto cleanup = [&]() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        if (cin) (void)hipStreamDestroy(cin);
+        if (cout) (void)hipStreamDestroy(cout);
+    };
+    auto run = [&]() -> int {
+        GPR_HIP(hipStreamCreateWithFlags(&cin, hipStreamNonBlocking));
+        GPR_HIP(hipStreamCreateWithFlags(&cout, hipStreamNonBlocking));
+        ev.assign(2 * nch, nullptr);
+        for (uint64_t k = 0; k < 2 * nch; ++k) GPR_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+        GPR_HIP(hipMemsetAsync(dov.p, 0, sizeof(unsigned long long), st));
+        auto copy_out = [&](uint64_t c) -> int {
+            const uint64_t off = c * kHostChunk, m = std::min(kHostChunk, n - off);
+            GPR_HIP(hipStreamWaitEvent(cout, ev[nch + c], 0));
+            GPR_HIP(hipMemcpyAsync(hits + off, dh + off, m * sizeof(gpr_hit), hipMemcpyDeviceToHost, cout));
+            return GPR_OK;
+        };
+        for (uint64_t c = 0; c < nch; ++c) {
+            const uint64_t off = c * kHostChunk, m = std::min(kHostChunk, n - off);
+            GPR_HIP(hipMemcpyAsync(dr + off, rays + off, m * sizeof(gpr_ray), hipMemcpyHostToDevice, cin));
+            GPR_HIP(hipEventRecord(ev[c], cin));
+            GPR_HIP(hipStreamWaitEvent(st, ev[c], 0));
+            GPR_TRY(launch_trace(sv, dr + off, dh + off, m, flags, nullptr, st, s->trace_nt, s->trace_sl,
+                                 s->trace_chunk, s->trace_refill));
+            GPR_TRY(trace_overflow_accumulate(dov.as<unsigned long long>(), st));
+            GPR_HIP(hipEventRecord(ev[nch + c], st));
+            if (c > 0) GPR_TRY(copy_out(c - 1));
+        }
+        GPR_TRY(copy_out(nch - 1));
+        unsigned long long ov = 0;
+        GPR_HIP(hipMemcpyAsync(&ov, dov.p, sizeof ov, hipMemcpyDeviceToHost, st));
+        GPR_HIP(hipStreamSynchronize(cout));
+        GPR_HIP(hipStreamSynchronize(st));
+        if (ov) return fail(GPR_ERR_INVALID, "traversal stack capacity exceeded on " + std::to_string(ov) + " rays");
+        return GPR_OK;
+    };
+    const int rc = run();
+    if (rc != GPR_OK) {
+        // nothing may still read or write the staging buffers when they are freed
+        if (cin) (void)hipStreamSynchronize(cin);
+        if (cout) (void)hipStreamSynchronize(cout);
+        (void)hipStreamSynchronize(st);
+    }
+    cleanup();
+    return rc;
+}
+}  // namespace
+
 int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags) {
     if (!s || !s->committed) return fail(GPR_ERR_INVALID, "gpr_trace: scene not committed");
     if (n == 0) return GPR_OK;
@@ -650,6 +718,8 @@ int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint
     DevBuf dr, dh;
     GPR_TRY(dr.alloc(n * sizeof(gpr_ray)));
     GPR_TRY(dh.alloc(n * sizeof(gpr_hit)));
+    if (!(flags & GPR_TRACE_REF_ORDER) && n > kHostChunk)
+        return trace_host_pipelined(s, sv, rays, hits, n, flags, dr.as<gpr_ray>(), dh.as<gpr_hit>());
     GPR_HIP(hipMemcpyAsync(dr.p, rays, n * sizeof(gpr_ray), hipMemcpyHostToDevice, stream()));
     GPR_TRY(launch_trace(sv, dr.as<gpr_ray>(), dh.as<gpr_hit>(), n, flags, stats, stream(), s->trace_nt,
                          s->trace_sl, s->trace_chunk, s->trace_refill));

@@ -17,6 +17,10 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
+__global__ void k_add_overflow(const unsigned long long* cur, unsigned long long* acc) {
+    if (threadIdx.x == 0) acc[0] += cur[1];
+}
+
 GPR_D Ray3 load_ray(const gpr_ray* rays, uint64_t i) {
     const float4* p = reinterpret_cast<const float4*>(rays + i);
     const float4 a = p[0], b = p[1];
@@ -279,6 +283,17 @@ int trace_overflow_count(unsigned long long* out) {
     if (R.dev < 0) return GPR_OK;
     GPR_HIP(hipMemcpyAsync(out, R.cursor.as<unsigned long long>() + 1, 8, hipMemcpyDeviceToHost, stream()));
     GPR_HIP(hipStreamSynchronize(stream()));
+    return GPR_OK;
+}
+
+// Adds the overflow count of the last fast trace launched on st to *acc
+// (device memory), in stream order, so that a batch traced in several launches
+// keeps the sum over its launches.
+int trace_overflow_accumulate(unsigned long long* acc, hipStream_t st) {
+    TraceResources& R = g_res[device_id() & 15];
+    if (R.dev < 0) return fail(GPR_ERR_INVALID, "trace_overflow_accumulate: no fast trace launched");
+    hipLaunchKernelGGL(k_add_overflow, dim3(1), dim3(64), 0, st, R.cursor.as<unsigned long long>(), acc);
+    GPR_HIP(hipGetLastError());
     return GPR_OK;
 }
 

@@ -261,7 +261,10 @@ typedef struct {
 #define GPR_TRACE_ANY 1u         /* any-hit: stop at the first accepted hit */
 #define GPR_TRACE_REF_ORDER 2u   /* reference visit order on the DFS layout */
 #define GPR_TRACE_DEVICE_PTRS 4u /* rays / hits are device pointers */
-/* trace / trace_shadow (src/renderer/pathtracer_kernel.h:235-246, 310-321). */
+/* trace / trace_shadow (src/renderer/pathtracer_kernel.h:235-246, 310-321).
+ * Host buffers (no GPR_TRACE_DEVICE_PTRS): the call stages the rays in and the
+ * hits out itself and returns when the hits are written; batches above 2^20
+ * rays move in chunks whose copies overlap the traversal of other chunks. */
 int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags);
 /* Autotune the BVH4 traversal of gpr_trace for this scene on a batch of device
  * rays (flags: GPR_TRACE_DEVICE_PTRS, optionally GPR_TRACE_ANY): block size
