@@ -10,7 +10,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gpr_internal.h"
@@ -613,19 +617,34 @@ int gpr_bvh_last_timing(const gpr_scene* s, float ms[5]) {
     return GPR_OK;
 }
 
+namespace {
+// Scratch of gpr_sort_pairs_device, one per (device, stream): sorts on one
+// stream run in order and share it; sorts on different streams may overlap.
+struct SortScratch {
+    DevBuf b;
+    uint32_t calls = 0;
+};
+std::mutex g_sort_mu;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<SortScratch>> g_sort_scratch;
+}  // namespace
+
 int gpr_sort_pairs_device(const uint32_t* keys, uint32_t* sk, uint32_t* si, uint32_t n) {
-    static DevBuf scratch[16];
-    static uint32_t calls[16];
     if (n == 0) return GPR_OK;
     if (!keys || !sk || !si) return fail(GPR_ERR_INVALID, "gpr_sort_pairs_device: null pointer");
-    DevBuf& b = scratch[g_device & 15];
-    const size_t need = sort_scratch_bytes(n);
-    if (!b.p || b.bytes < need) {
-        GPR_TRY(b.alloc(need));
-        GPR_TRY(sort_prepare(b.p, b.bytes, stream()));
-        calls[g_device & 15] = 0;
+    SortScratch* sc = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_sort_mu);
+        std::unique_ptr<SortScratch>& p = g_sort_scratch[{g_device, stream()}];
+        if (!p) p = std::make_unique<SortScratch>();
+        sc = p.get();
     }
-    return sort_pairs(keys, sk, si, n, b.p, b.bytes, &calls[g_device & 15], stream());
+    const size_t need = sort_scratch_bytes(n);
+    if (!sc->b.p || sc->b.bytes < need) {
+        GPR_TRY(sc->b.alloc(need));
+        GPR_TRY(sort_prepare(sc->b.p, sc->b.bytes, stream()));
+        sc->calls = 0;
+    }
+    return sort_pairs(keys, sk, si, n, sc->b.p, sc->b.bytes, &sc->calls, stream());
 }
 
 namespace {

@@ -6,7 +6,11 @@
 // the device (not to the batch) and the per-lane overflow stacks are indexed
 // by the persistent thread id.
 #include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "traverse.h"
 
@@ -216,7 +220,18 @@ struct TraceResources {
     int grid[kTraceVariants][4] = {};  // [variant][closest, any, closest refill, any refill]
     int dev = -1;
 };
-TraceResources g_res[16];
+// Work buffers of the fast traces, one set per (device, stream): launches on
+// one stream run in order and may share a ray cursor and a stack spill area;
+// launches on different streams may overlap, so each stream has its own.  The
+// sets live until the process ends.
+std::mutex g_res_mu;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<TraceResources>> g_res;
+
+TraceResources* find_resources(hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    auto it = g_res.find({device_id(), st});
+    return it == g_res.end() ? nullptr : it->second.get();
+}
 
 int ensure_resources(TraceResources& R) {
     const int dev = device_id();
@@ -237,6 +252,15 @@ int ensure_resources(TraceResources& R) {
     GPR_TRY(R.spill.alloc(threads * kStackGlobal * sizeof(int2)));
     GPR_TRY(R.cursor.alloc(4 * sizeof(unsigned long long)));
     R.dev = dev;
+    return GPR_OK;
+}
+
+int resources_for(hipStream_t st, TraceResources** out) {
+    std::lock_guard<std::mutex> lk(g_res_mu);
+    std::unique_ptr<TraceResources>& p = g_res[{device_id(), st}];
+    if (!p) p = std::make_unique<TraceResources>();
+    GPR_TRY(ensure_resources(*p));
+    *out = p.get();
     return GPR_OK;
 }
 
@@ -261,8 +285,9 @@ int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64
     }
     const int k = trace_variant_index(nt, sl);
     if (k < 0 || !trace_cfg_valid(nt, sl, chunk)) return fail(GPR_ERR_INVALID, "trace: configuration outside the tuned set");
-    TraceResources& R = g_res[device_id() & 15];
-    GPR_TRY(ensure_resources(R));
+    TraceResources* rp = nullptr;
+    GPR_TRY(resources_for(st, &rp));
+    TraceResources& R = *rp;
     const TraceVariant& v = trace_variant_at(k);
     unsigned long long* cur = R.cursor.as<unsigned long long>();
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cur, 4);
@@ -276,12 +301,13 @@ int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64
     return GPR_OK;
 }
 
-// Overflow count of the last fast trace on this device (0 = every ray completed).
+// Overflow count of the last fast trace on this thread's device and stream
+// (0 = every ray completed).
 int trace_overflow_count(unsigned long long* out) {
-    TraceResources& R = g_res[device_id() & 15];
     *out = 0;
-    if (R.dev < 0) return GPR_OK;
-    GPR_HIP(hipMemcpyAsync(out, R.cursor.as<unsigned long long>() + 1, 8, hipMemcpyDeviceToHost, stream()));
+    const TraceResources* R = find_resources(stream());
+    if (!R || R->dev < 0) return GPR_OK;
+    GPR_HIP(hipMemcpyAsync(out, R->cursor.as<unsigned long long>() + 1, 8, hipMemcpyDeviceToHost, stream()));
     GPR_HIP(hipStreamSynchronize(stream()));
     return GPR_OK;
 }
@@ -290,9 +316,9 @@ int trace_overflow_count(unsigned long long* out) {
 // (device memory), in stream order, so that a batch traced in several launches
 // keeps the sum over its launches.
 int trace_overflow_accumulate(unsigned long long* acc, hipStream_t st) {
-    TraceResources& R = g_res[device_id() & 15];
-    if (R.dev < 0) return fail(GPR_ERR_INVALID, "trace_overflow_accumulate: no fast trace launched");
-    hipLaunchKernelGGL(k_add_overflow, dim3(1), dim3(64), 0, st, R.cursor.as<unsigned long long>(), acc);
+    const TraceResources* R = find_resources(st);
+    if (!R || R->dev < 0) return fail(GPR_ERR_INVALID, "trace_overflow_accumulate: no fast trace launched");
+    hipLaunchKernelGGL(k_add_overflow, dim3(1), dim3(64), 0, st, R->cursor.as<unsigned long long>(), acc);
     GPR_HIP(hipGetLastError());
     return GPR_OK;
 }

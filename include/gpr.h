@@ -243,7 +243,10 @@ int gpr_bvh_last_timing(const gpr_scene* s, float ms[5]);
 
 /* Standalone LDS-staged radix sort of n Morton keys into (key, index) pairs,
  * stable (ties in index order), the sort of LBVH_build (bvh.h:207-208).
- * Device pointers; keys must be below 2^30 (30-bit Morton codes). */
+ * Device pointers; keys must be below 2^30 (30-bit Morton codes).  Its
+ * scratch belongs to the calling thread's (device, stream) and grows on the
+ * first call with a larger n: to capture it into a hipGraph, call it once on
+ * the capture stream with that n beforehand, outside the capture. */
 int gpr_sort_pairs_device(const uint32_t* keys, uint32_t* sorted_keys, uint32_t* sorted_idx,
                           uint32_t n);
 
@@ -262,6 +265,9 @@ typedef struct {
 #define GPR_TRACE_REF_ORDER 2u   /* reference visit order on the DFS layout */
 #define GPR_TRACE_DEVICE_PTRS 4u /* rays / hits are device pointers */
 /* trace / trace_shadow (src/renderer/pathtracer_kernel.h:235-246, 310-321).
+ * The BVH4 traversal's work buffers (a ray cursor and a stack spill area of
+ * 768 B per resident lane) belong to the calling thread's (device, stream):
+ * traces on different streams may run concurrently.
  * Host buffers (no GPR_TRACE_DEVICE_PTRS): the call stages the rays in and the
  * hits out itself and returns when the hits are written; batches above 2^20
  * rays move in chunks whose copies overlap the traversal of other chunks. */

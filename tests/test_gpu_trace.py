@@ -92,6 +92,55 @@ def test_host_buffers_in_chunks_equal_device_buffers():
             assert (h["id"] != -1).mean() > 0.05
 
 
+def test_concurrent_streams_trace_and_sort_as_alone():
+    # two host threads, each on its own stream, trace different scenes and
+    # sort different keys at the same time (the traversal cursor / spill area
+    # and the sort scratch are kept per stream); every result equals the same
+    # work done alone
+    import threading
+    import torch
+    n = 1 << 18
+    scenes = [G.scene_from_soup(G.soup(1 << 15, seed=k)) for k in (11, 12)]
+    d_r = [torch.from_numpy(rays_for(n, seed=k).view(np.uint8)).cuda() for k in (11, 12)]
+    d_h = [torch.empty(n * 16, dtype=torch.uint8, device="cuda") for _ in range(2)]
+    rng = np.random.default_rng(8)
+    keys = [rng.integers(0, 1 << 30, 300001 + 77 * k).astype(np.uint32) for k in range(2)]
+    d_k = [torch.from_numpy(k.view(np.int32)).cuda() for k in keys]
+    d_sk = [torch.empty_like(k) for k in d_k]
+    d_si = [torch.empty_like(k) for k in d_k]
+    alone = []
+    for k in range(2):
+        scenes[k].trace_device(d_r[k].data_ptr(), d_h[k].data_ptr(), n)
+        torch.cuda.synchronize()
+        alone.append(d_h[k].cpu().numpy().copy())
+    streams = [torch.cuda.Stream() for _ in range(2)]
+    torch.cuda.synchronize()
+    errors = []
+
+    def work(k):
+        try:
+            gpr.set_stream(streams[k].cuda_stream)
+            for _ in range(12):
+                scenes[k].trace_device(d_r[k].data_ptr(), d_h[k].data_ptr(), n)
+                gpr.sort_pairs_device(d_k[k].data_ptr(), d_sk[k].data_ptr(), d_si[k].data_ptr(), len(keys[k]))
+            streams[k].synchronize()
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    print("[trace] two streams from two threads", flush=True)
+    th = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    for k in range(2):
+        assert np.array_equal(d_h[k].cpu().numpy(), alone[k])
+        osk, osi = O.sort_pairs(keys[k])
+        assert np.array_equal(d_sk[k].cpu().numpy().view(np.uint32), osk)
+        assert np.array_equal(d_si[k].cpu().numpy().view(np.uint32), osi)
+
+
 def test_fast_vs_fp64_bruteforce_1m():
     pos = G.soup(1 << 20)
     s = G.scene_from_soup(pos)
