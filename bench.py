@@ -377,8 +377,7 @@ def main():
         print(json.dumps({"profiled": "closest-hit", "launches": a.warmup + a.steps}), flush=True)
         return
     # ---- radix sort: mean over back-to-back sorts of the scene's Morton keys,
-    # captured as one graph of sort_iters sorts (an even count keeps the
-    # double-buffered digit totals in phase) and replayed
+    # captured as one graph of sort_iters sorts and replayed
     keys, _, _ = scene.keys()
     d_keys = torch.from_numpy(keys.view(np.int32)).cuda()
     d_sk = torch.empty_like(d_keys)
@@ -386,7 +385,7 @@ def main():
     gpr.sort_pairs_device(d_keys.data_ptr(), d_sk.data_ptr(), d_si.data_ptr(), N_TRI)
     torch.cuda.synchronize()
     ref_sk, ref_si = d_sk.clone(), d_si.clone()
-    iters = a.sort_iters + (a.sort_iters & 1)
+    iters = max(1, a.sort_iters)
     graph = torch.cuda.CUDAGraph()
     cap = torch.cuda.Stream()
     with torch.cuda.stream(cap):

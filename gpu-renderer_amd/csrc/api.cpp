@@ -622,7 +622,6 @@ namespace {
 // stream run in order and share it; sorts on different streams may overlap.
 struct SortScratch {
     DevBuf b;
-    uint32_t calls = 0;
 };
 std::mutex g_sort_mu;
 std::map<std::pair<int, hipStream_t>, std::unique_ptr<SortScratch>> g_sort_scratch;
@@ -642,9 +641,8 @@ int gpr_sort_pairs_device(const uint32_t* keys, uint32_t* sk, uint32_t* si, uint
     if (!sc->b.p || sc->b.bytes < need) {
         GPR_TRY(sc->b.alloc(need));
         GPR_TRY(sort_prepare(sc->b.p, sc->b.bytes, stream()));
-        sc->calls = 0;
     }
-    return sort_pairs(keys, sk, si, n, sc->b.p, sc->b.bytes, &sc->calls, stream());
+    return sort_pairs(keys, sk, si, n, sc->b.p, sc->b.bytes, stream());
 }
 
 namespace {

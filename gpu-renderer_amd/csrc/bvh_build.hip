@@ -492,7 +492,6 @@ int build_bvh(gpr_scene* s) {
             if (B.sort_scratch.bytes < sb || !B.sort_scratch.p) {
                 GPR_TRY(B.sort_scratch.alloc(sb));
                 GPR_TRY(sort_prepare(B.sort_scratch.p, B.sort_scratch.bytes, st));
-                B.sort_calls = 0;
             }
             continue;
         }
@@ -510,7 +509,7 @@ int build_bvh(gpr_scene* s) {
         hipLaunchKernelGGL(k_centroid_final, dim3(1), dim3(kB), 0, st, part, npart, bounds);
         hipLaunchKernelGGL(k_morton, dim3(div_up(n, kB)), dim3(kB), 0, st, pos, n, bounds, keys);
         GPR_HIP(hipEventRecord(ev[1], st));
-        GPR_TRY(sort_pairs(keys, skeys, sidx, n, B.sort_scratch.p, B.sort_scratch.bytes, &B.sort_calls, st));
+        GPR_TRY(sort_pairs(keys, skeys, sidx, n, B.sort_scratch.p, B.sort_scratch.bytes, st));
         GPR_HIP(hipEventRecord(ev[2], st));
         hipLaunchKernelGGL(k_fill_u32, dim3(std::min(1024u, div_up(n, kB))), dim3(kB), 0, st, arrive, 0u, (uint64_t)n);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kB), 0, st, root + 1, 0u, (uint64_t)1);

@@ -140,7 +140,6 @@ struct BvhDev {
     int32_t root4 = 0;
     DevBuf arena;      // all per-build scratch and results
     DevBuf sort_scratch;
-    uint32_t sort_calls = 0;
     uint32_t* keys = nullptr;
     uint32_t* skeys = nullptr;
     uint32_t* sidx = nullptr;
@@ -200,7 +199,7 @@ namespace gpr {
 int build_bvh(gpr_scene* s);
 // sort.hip
 int sort_pairs(const uint32_t* keys, uint32_t* skeys, uint32_t* sidx, uint32_t n, void* scratch,
-               size_t scratch_bytes, uint32_t* calls, hipStream_t st);
+               size_t scratch_bytes, hipStream_t st);
 size_t sort_scratch_bytes(uint32_t n);
 int sort_prepare(void* scratch, size_t bytes, hipStream_t st);
 // trace.hip

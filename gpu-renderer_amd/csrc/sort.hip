@@ -21,8 +21,9 @@
 //            composite is placed by its rank among its sub-bucket.  Buckets
 //            with a crowded sub-bucket, or too large for LDS, use LDS bitonic
 //            chunk sorts and pairwise merges.
-// The digit-total buffer is double-buffered across calls: each call clears
-// the one the next call uses.
+// Every bucket block zeroes its digit total once it has read it, so each call
+// leaves the totals cleared for the next call in stream order: eager calls and
+// replays of captured graphs (of any number of sorts) may interleave freely.
 #include "dev_math.h"
 #include "gpr_internal.h"
 
@@ -93,12 +94,9 @@ GPR_D uint32_t scan_incl(uint32_t v, uint32_t* sh, uint32_t* total) {
 __global__ __launch_bounds__(kTileThreads) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                        uint32_t shift, uint32_t nd,
                                                        uint32_t* __restrict__ acc,
-                                                       uint32_t* __restrict__ acc_next,
                                                        uint32_t* __restrict__ off) {
     __shared__ uint32_t h[kQuarters * kMaxDigits];
     for (uint32_t i = threadIdx.x; i < kQuarters * nd; i += kTileThreads) h[i] = 0;
-    if (blockIdx.x == 0)
-        for (uint32_t d = threadIdx.x; d < nd; d += kTileThreads) acc_next[d] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kTile;
     uint32_t k[kTile / kTileThreads];
@@ -263,7 +261,7 @@ GPR_D uint32_t lower_bound_u64(const unsigned long long* a, uint32_t n, unsigned
 }
 
 // ---- local: one bucket per block, LDS MSD split + rank inside each sub-bucket
-__global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const uint32_t* __restrict__ acc,
+__global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, uint32_t* __restrict__ acc,
                                                          const uint32_t* __restrict__ bases,
                                                          unsigned long long* __restrict__ tmp,
                                                          unsigned long long* __restrict__ alt,
@@ -277,6 +275,8 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const u
     const uint32_t d = blockIdx.x;
     const uint32_t n = acc[d];
     const uint32_t b0 = bases[d];
+    __syncthreads();  // every lane has read the bucket's total ...
+    if (threadIdx.x == 0) acc[d] = 0u;  // ... which the next call starts from zero
     if (n == 0) return;
     unsigned long long* src = tmp + b0;
     const uint32_t sub_bits = min(10u, shift);
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, const u
 }
 
 struct Layout {
-    uint32_t* acc[2];
+    uint32_t* acc;
     uint32_t* bases;
     uint32_t* off;
     unsigned long long* tmp;
@@ -409,8 +409,7 @@ size_t layout(uint32_t n, char* base, Layout* L) {
         return p;
     };
     Layout l;
-    l.acc[0] = reinterpret_cast<uint32_t*>(take(4 * kMaxDigits));
-    l.acc[1] = reinterpret_cast<uint32_t*>(take(4 * kMaxDigits));
+    l.acc = reinterpret_cast<uint32_t*>(take(4 * kMaxDigits));
     l.bases = reinterpret_cast<uint32_t*>(take(4 * kMaxDigits));
     l.off = reinterpret_cast<uint32_t*>(take(4ull * kMaxDigits * kQuarters * ntiles));
     l.tmp = reinterpret_cast<unsigned long long*>(take(8ull * std::max(1u, n)));
@@ -424,28 +423,27 @@ size_t layout(uint32_t n, char* base, Layout* L) {
 size_t sort_scratch_bytes(uint32_t n) { return layout(n, nullptr, nullptr); }
 
 int sort_prepare(void* scratch, size_t bytes, hipStream_t st) {
-    // both digit-total buffers start cleared; afterwards every call clears the next one
-    GPR_HIP(hipMemsetAsync(scratch, 0, std::min<size_t>(bytes, 2 * 4 * kMaxDigits + 512), st));
+    // the digit totals (the first array of the layout) start cleared; afterwards
+    // every call clears them after use
+    GPR_HIP(hipMemsetAsync(scratch, 0, std::min<size_t>(bytes, 4 * kMaxDigits), st));
     return GPR_OK;
 }
 
 int sort_pairs(const uint32_t* keys, uint32_t* skeys, uint32_t* sidx, uint32_t n, void* scratch,
-               size_t scratch_bytes, uint32_t* calls, hipStream_t st) {
+               size_t scratch_bytes, hipStream_t st) {
     if (n == 0) return GPR_OK;
     if (scratch_bytes < sort_scratch_bytes(n)) return fail(GPR_ERR_INVALID, "sort scratch too small");
     Layout L;
     layout(n, reinterpret_cast<char*>(scratch), &L);
-    const uint32_t par = (*calls)++ & 1u;
     const uint32_t dbits = digit_bits(n);
     const uint32_t nd = 1u << dbits, shift = 30 - dbits;
     const uint32_t ntiles = div_up(n, kTile);
-    hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kTileThreads), 0, st, keys, n, shift, nd, L.acc[par],
-                       L.acc[par ^ 1], L.off);
+    hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kTileThreads), 0, st, keys, n, shift, nd, L.acc, L.off);
     // grid padded to whole groups of kXcds tiles for the XCD mapping
     const uint32_t nsub_blocks = div_up(ntiles, kXcds) * kXcds * kQuarters;
     hipLaunchKernelGGL(k_scatter, dim3(nsub_blocks), dim3(kTileThreads), 0, st, keys, n, shift, nd,
-                       L.acc[par], L.off, L.bases, L.tmp);
-    hipLaunchKernelGGL(k_local, dim3(nd), dim3(kLocalThreads), 0, st, shift, L.acc[par], L.bases, L.tmp,
+                       L.acc, L.off, L.bases, L.tmp);
+    hipLaunchKernelGGL(k_local, dim3(nd), dim3(kLocalThreads), 0, st, shift, L.acc, L.bases, L.tmp,
                        L.alt, skeys, sidx);
     GPR_HIP(hipGetLastError());
     return GPR_OK;

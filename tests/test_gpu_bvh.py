@@ -78,6 +78,50 @@ def test_sort_device_entry_point():
         assert np.array_equal(si.cpu().numpy().view(np.uint32), osi)
 
 
+def test_sort_graph_replays_interleaved_with_eager_sorts():
+    # three sorts captured into one graph (an odd count) and replayed three
+    # times, with an eager sort of other keys (another digit count) on the same
+    # stream and synchronisations between the replays: every replay and every
+    # eager sort matches the oracle (each sort leaves its digit totals cleared
+    # for the next one in stream order)
+    import torch
+    rng = np.random.default_rng(21)
+    ka = rng.integers(0, 1 << 30, 300001).astype(np.uint32)
+    kb = rng.integers(0, 1 << 30, 100003).astype(np.uint32)
+    osa, osb = O.sort_pairs(ka), O.sort_pairs(kb)
+    d_a = torch.from_numpy(ka.view(np.int32)).cuda()
+    d_b = torch.from_numpy(kb.view(np.int32)).cuda()
+    sa, ia, sb, ib = (torch.empty_like(d_a), torch.empty_like(d_a), torch.empty_like(d_b), torch.empty_like(d_b))
+
+    def check(sk, si, ref, label):
+        assert np.array_equal(sk.cpu().numpy().view(np.uint32), ref[0]), label
+        assert np.array_equal(si.cpu().numpy().view(np.uint32), ref[1]), label
+
+    st = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    print("[bvh] sort graph replays", flush=True)
+    try:
+        with torch.cuda.stream(st):
+            gpr.set_stream(st.cuda_stream)
+            # allocates this stream's scratch for the larger n, outside the capture
+            gpr.sort_pairs_device(d_a.data_ptr(), sa.data_ptr(), ia.data_ptr(), len(ka))
+            torch.cuda.synchronize()
+            with torch.cuda.graph(graph, stream=st):
+                for _ in range(3):
+                    gpr.sort_pairs_device(d_a.data_ptr(), sa.data_ptr(), ia.data_ptr(), len(ka))
+            for rep in range(3):
+                sa.zero_()
+                ia.zero_()
+                graph.replay()
+                torch.cuda.synchronize()
+                check(sa, ia, osa, f"replay {rep}")
+                gpr.sort_pairs_device(d_b.data_ptr(), sb.data_ptr(), ib.data_ptr(), len(kb))
+                torch.cuda.synchronize()
+                check(sb, ib, osb, f"eager {rep}")
+    finally:
+        gpr.set_stream(0)
+
+
 def test_empty_scene():
     s = gpr.Scene().commit()
     assert s.info().ntri == 0
