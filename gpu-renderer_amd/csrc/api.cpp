@@ -657,10 +657,9 @@ constexpr uint64_t kHostChunk = 1ull << 20;
 // memory can block the calling thread until it completes, and this order keeps
 // the device busy meanwhile.  Results are those of one launch over the batch.
 int trace_host_pipelined(gpr_scene* s, const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64_t n,
-                         uint32_t flags, gpr_ray* dr, gpr_hit* dh) {
+                         uint32_t flags, gpr_ray* dr, gpr_hit* dh, DevBuf& dov) {
     hipStream_t st = stream();
     const uint64_t nch = (n + kHostChunk - 1) / kHostChunk;
-    DevBuf dov;
     GPR_TRY(dov.alloc(sizeof(unsigned long long)));
     hipStream_t cin = nullptr, cout = nullptr;
     std::vector<hipEvent_t> ev;
@@ -710,6 +709,24 @@ int trace_host_pipelined(gpr_scene* s, const SceneView& sv, const gpr_ray* rays,
     cleanup();
     return rc;
 }
+
+// Device staging of host-buffer traces, one set per (device, stream), grown as
+// needed and kept until the process ends.  A call holds its set's lock until
+// its copies are done, so host-buffer traces on one stream from several
+// threads take turns.
+struct TraceStaging {
+    std::mutex mu;
+    DevBuf rays, hits, ov;
+};
+std::mutex g_stage_mu;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<TraceStaging>> g_stage;
+
+TraceStaging* staging_for_stream() {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    std::unique_ptr<TraceStaging>& p = g_stage[{g_device, stream()}];
+    if (!p) p = std::make_unique<TraceStaging>();
+    return p.get();
+}
 }  // namespace
 
 int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags) {
@@ -732,11 +749,14 @@ int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint
     if (flags & GPR_TRACE_DEVICE_PTRS)
         return launch_trace(sv, rays, hits, n, flags, stats, stream(), s->trace_nt, s->trace_sl, s->trace_chunk,
                             s->trace_refill);
-    DevBuf dr, dh;
+    TraceStaging* sg = staging_for_stream();
+    std::lock_guard<std::mutex> lk(sg->mu);
+    DevBuf& dr = sg->rays;
+    DevBuf& dh = sg->hits;
     GPR_TRY(dr.alloc(n * sizeof(gpr_ray)));
     GPR_TRY(dh.alloc(n * sizeof(gpr_hit)));
     if (!(flags & GPR_TRACE_REF_ORDER) && n > kHostChunk)
-        return trace_host_pipelined(s, sv, rays, hits, n, flags, dr.as<gpr_ray>(), dh.as<gpr_hit>());
+        return trace_host_pipelined(s, sv, rays, hits, n, flags, dr.as<gpr_ray>(), dh.as<gpr_hit>(), sg->ov);
     GPR_HIP(hipMemcpyAsync(dr.p, rays, n * sizeof(gpr_ray), hipMemcpyHostToDevice, stream()));
     GPR_TRY(launch_trace(sv, dr.as<gpr_ray>(), dh.as<gpr_hit>(), n, flags, stats, stream(), s->trace_nt,
                          s->trace_sl, s->trace_chunk, s->trace_refill));

@@ -270,7 +270,9 @@ typedef struct {
  * traces on different streams may run concurrently.
  * Host buffers (no GPR_TRACE_DEVICE_PTRS): the call stages the rays in and the
  * hits out itself and returns when the hits are written; batches above 2^20
- * rays move in chunks whose copies overlap the traversal of other chunks. */
+ * rays move in chunks whose copies overlap the traversal of other chunks.  The
+ * device staging (64 B per ray) is kept per (device, stream) until the process
+ * ends; host-buffer calls on one stream from several threads take turns. */
 int gpr_trace(gpr_scene* s, const gpr_ray* rays, gpr_hit* hits, uint64_t n, uint32_t flags);
 /* Autotune the BVH4 traversal of gpr_trace for this scene on a batch of device
  * rays (flags: GPR_TRACE_DEVICE_PTRS, optionally GPR_TRACE_ANY): block size
