@@ -28,6 +28,8 @@
 #include "gpr_internal.h"
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace gpr {
 namespace {
@@ -399,23 +401,39 @@ struct Layout {
     unsigned long long* tmp;
     unsigned long long* alt;
 };
+// Byte offsets of the scratch arrays for n keys, each on a 256-B boundary
+// (constexpr: the no-allocation entry point sizes its fixed array with it).
+struct Offsets {
+    size_t acc, bases, off, tmp, alt, total;
+};
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr Offsets offsets(uint32_t n) {
+    const uint64_t ntiles = n > (uint32_t)kTile ? ((uint64_t)n + kTile - 1) / kTile : 1;
+    const uint64_t nk = n ? n : 1;
+    Offsets o{};
+    o.acc = 0;
+    o.bases = align256(o.acc + 4ull * kMaxDigits);
+    o.off = align256(o.bases + 4ull * kMaxDigits);
+    o.tmp = align256(o.off + 4ull * kMaxDigits * kQuarters * ntiles);
+    o.alt = align256(o.tmp + 8ull * nk);
+    o.total = o.alt + 8ull * nk + 256;
+    return o;
+}
 size_t layout(uint32_t n, char* base, Layout* L) {
-    const uint32_t ntiles = std::max(1u, div_up(n, kTile));
-    size_t off = 0;
-    auto take = [&](size_t bytes) {
-        off = (off + 255) & ~(size_t)255;
-        char* p = base ? base + off : nullptr;
-        off += bytes;
-        return p;
-    };
-    Layout l;
-    l.acc = reinterpret_cast<uint32_t*>(take(4 * kMaxDigits));
-    l.bases = reinterpret_cast<uint32_t*>(take(4 * kMaxDigits));
-    l.off = reinterpret_cast<uint32_t*>(take(4ull * kMaxDigits * kQuarters * ntiles));
-    l.tmp = reinterpret_cast<unsigned long long*>(take(8ull * std::max(1u, n)));
-    l.alt = reinterpret_cast<unsigned long long*>(take(8ull * std::max(1u, n)));
-    if (L) *L = l;
-    return off + 256;
+    const Offsets o = offsets(n);
+    if (L) {
+        L->acc = reinterpret_cast<uint32_t*>(base + o.acc);
+        L->bases = reinterpret_cast<uint32_t*>(base + o.bases);
+        L->off = reinterpret_cast<uint32_t*>(base + o.off);
+        L->tmp = reinterpret_cast<unsigned long long*>(base + o.tmp);
+        L->alt = reinterpret_cast<unsigned long long*>(base + o.alt);
+    }
+    return o.total;
+}
+
+// zero fill of n words (graph-capturable, unlike a memset node)
+__global__ __launch_bounds__(1024) void k_clear_words(uint32_t* __restrict__ p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
 }  // namespace
@@ -450,3 +468,58 @@ int sort_pairs(const uint32_t* keys, uint32_t* skeys, uint32_t* sidx, uint32_t n
 }
 
 }  // namespace gpr
+
+// ---- the sort without allocation ------------------------------------------
+// A fixed device array holds the scratch of sorts of up to kNoAllocMaxKeys
+// keys, so gpr_sort_pairs_device_noalloc calls neither hipMalloc nor hipFree.
+// Each GPU holds its own copy of the array until the process ends (18.9 MB);
+// every call on a GPU shares it.
+namespace {
+constexpr uint32_t kNoAllocMaxKeys = 1u << 20;
+constexpr size_t kNoAllocBytes = gpr::offsets(kNoAllocMaxKeys).total;
+constexpr int kMaxDevices = 16;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+// The array's device address on the current device, looked up once per device
+// with that device current.  In host code `scratch` names a host-side stand-in,
+// never the device memory: only this address is passed to kernels.
+int noalloc_scratch(void** out) {
+    int dev = 0;
+    GPR_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return gpr::fail(GPR_ERR_INVALID, "gpr_sort_pairs_device_noalloc: device index outside 0..15");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        GPR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return gpr::fail(GPR_ERR_HIP, "gpr_sort_pairs_device_noalloc: no device address for the scratch array");
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return GPR_OK;
+}
+}  // namespace
+
+int gpr_sort_pairs_device_noalloc(const uint32_t* keys, uint32_t* sk, uint32_t* si, uint32_t n) {
+    using namespace gpr;
+    if (n == 0) return GPR_OK;
+    if (!keys || !sk || !si) return fail(GPR_ERR_INVALID, "gpr_sort_pairs_device_noalloc: null pointer");
+    if (n > kNoAllocMaxKeys)
+        return fail(GPR_ERR_INVALID, "gpr_sort_pairs_device_noalloc: n = " + std::to_string(n) +
+                                         " exceeds the fixed scratch's maximum of " +
+                                         std::to_string(kNoAllocMaxKeys) + " keys");
+    if (sort_scratch_bytes(n) > kNoAllocBytes)
+        return fail(GPR_ERR_INVALID, "gpr_sort_pairs_device_noalloc: scratch layout exceeds the fixed array");
+    void* p = nullptr;
+    GPR_TRY(noalloc_scratch(&p));
+    // the digit totals start at zero whatever an earlier call left in the array
+    hipLaunchKernelGGL(k_clear_words, dim3(2), dim3(1024), 0, stream(), reinterpret_cast<uint32_t*>(p),
+                       (uint32_t)kMaxDigits);
+    GPR_HIP(hipGetLastError());
+    return sort_pairs(keys, sk, si, n, p, kNoAllocBytes, stream());
+}

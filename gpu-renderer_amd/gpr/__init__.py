@@ -154,6 +154,7 @@ _SIG = {
     "gpr_bvh_export_ref_layout": (C.c_int, [_P, _P, C.c_size_t]),
     "gpr_bvh_last_timing": (C.c_int, [_P, _P]),
     "gpr_sort_pairs_device": (C.c_int, [_P, _P, _P, C.c_uint32]),
+    "gpr_sort_pairs_device_noalloc": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "gpr_trace": (C.c_int, [_P, _P, _P, C.c_uint64, C.c_uint32]),
     "gpr_trace_tune": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.POINTER(TraceCfg), C.c_uint32,
                                  C.POINTER(C.c_uint32)]),
@@ -701,6 +702,12 @@ class Server:
 
 def sort_pairs_device(keys_ptr, skeys_ptr, sidx_ptr, n):
     check(lib.gpr_sort_pairs_device(C.c_void_p(keys_ptr), C.c_void_p(skeys_ptr), C.c_void_p(sidx_ptr), n))
+
+
+def sort_pairs_device_noalloc(keys_ptr, skeys_ptr, sidx_ptr, n):
+    """gpr_sort_pairs_device with the library's fixed scratch array (no
+    allocation; n at most 1,048,576)."""
+    check(lib.gpr_sort_pairs_device_noalloc(C.c_void_p(keys_ptr), C.c_void_p(skeys_ptr), C.c_void_p(sidx_ptr), n))
 
 
 def set_stream(ptr):

@@ -249,6 +249,16 @@ int gpr_bvh_last_timing(const gpr_scene* s, float ms[5]);
  * the capture stream with that n beforehand, outside the capture. */
 int gpr_sort_pairs_device(const uint32_t* keys, uint32_t* sorted_keys, uint32_t* sorted_idx,
                           uint32_t n);
+/* gpr_sort_pairs_device without allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as gpr_sort_pairs_device; its scratch is a fixed
+ * device array of 18.9 MB per GPU, held until the process ends, so n is at
+ * most 1,048,576 (larger n fails with GPR_ERR_INVALID).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (different streams or
+ * threads without ordering between them) race and give wrong results. */
+int gpr_sort_pairs_device_noalloc(const uint32_t* keys, uint32_t* sorted_keys, uint32_t* sorted_idx,
+                                  uint32_t n);
 
 /* -------------------------------------------------------------- trace --- */
 /* Traversal launch configuration picked by the autotuners (gpr_trace_tune,
