@@ -1580,6 +1580,19 @@ float orc_ggx_d(const float wh[3], float ax, float ay) { return tr_d(ld3(wh), ax
 float orc_ggx_g(const float wo[3], const float wi[3], float ax, float ay) {
     return tr_g(ld3(wo), ld3(wi), ax, ay);
 }
+float orc_ggx_lambda(const float w[3], float ax, float ay) { return tr_lambda(ld3(w), ax, ay); }
+void orc_ggx_d_n(const float* wh, uint32_t n, float ax, float ay, float* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = tr_d(ld3(wh + 3 * i), ax, ay);
+}
+void orc_ggx_sample_wh(const float wo[3], const float* u, uint32_t n, float ax, float ay, float* wh) {
+    const f3 w = ld3(wo);
+    for (uint32_t i = 0; i < n; ++i) {
+        const f3 h = sample_wh(w, u[2 * i], u[2 * i + 1], ax, ay);
+        wh[3 * i] = h.x;
+        wh[3 * i + 1] = h.y;
+        wh[3 * i + 2] = h.z;
+    }
+}
 
 int orc_render(const orc_scene* scene, const orc_node* nodes, const orc_render_opts* opts,
                float* rgb_sum, uint32_t* sample_num, int nthreads) {

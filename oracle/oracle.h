@@ -153,6 +153,13 @@ void orc_fr_conductor(float cos_i, const float eta_i[3], const float eta_t[3],
                       const float k[3], float out[3]);
 float orc_ggx_d(const float wh[3], float ax, float ay);
 float orc_ggx_g(const float wo[3], const float wi[3], float ax, float ay);
+/* Smith Lambda of the Trowbridge-Reitz distribution (microfacet.h:7-19). */
+float orc_ggx_lambda(const float w[3], float ax, float ay);
+/* D at n half vectors (wh: 3n floats). */
+void orc_ggx_d_n(const float* wh, uint32_t n, float ax, float ay, float* out);
+/* Sample_wh (microfacet.h:41-130, pbrt-v3's visible-normal sampler) for n
+ * uniform pairs u[2i], u[2i+1]; wh receives 3n floats. */
+void orc_ggx_sample_wh(const float wo[3], const float* u, uint32_t n, float ax, float ay, float* wh);
 
 /* --- integrator (pathtracer.cu:41-150, pathtracer_kernel.h:59-790) --- */
 int orc_render(const orc_scene* scene, const orc_node* nodes,

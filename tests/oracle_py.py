@@ -86,6 +86,12 @@ lib.orc_ggx_d.restype = C.c_float
 lib.orc_ggx_d.argtypes = [P, C.c_float, C.c_float]
 lib.orc_ggx_g.restype = C.c_float
 lib.orc_ggx_g.argtypes = [P, P, C.c_float, C.c_float]
+lib.orc_ggx_lambda.restype = C.c_float
+lib.orc_ggx_lambda.argtypes = [P, C.c_float, C.c_float]
+lib.orc_ggx_d_n.restype = None
+lib.orc_ggx_d_n.argtypes = [P, C.c_uint32, C.c_float, C.c_float, P]
+lib.orc_ggx_sample_wh.restype = None
+lib.orc_ggx_sample_wh.argtypes = [P, P, C.c_uint32, C.c_float, C.c_float, P]
 lib.orc_tex_eval.restype = None
 lib.orc_tex_eval.argtypes = [C.POINTER(Scene), C.c_uint32, C.c_float, C.c_float, P]
 lib.orc_env_le.restype = None
