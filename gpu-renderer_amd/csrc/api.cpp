@@ -839,6 +839,59 @@ int gpr_trace_ref_stats(const gpr_scene* s, uint64_t* nv, uint64_t* nt) {
     return GPR_OK;
 }
 
+int gpr_trace_bvh4_stats(gpr_scene* s, const gpr_ray* rays, uint64_t n, uint32_t flags, uint64_t* node_fetches,
+                         uint64_t* tri_tests) {
+    if (!s || !s->committed) return fail(GPR_ERR_INVALID, "gpr_trace_bvh4_stats: scene not committed");
+    if (n && !rays) return fail(GPR_ERR_INVALID, "gpr_trace_bvh4_stats: null rays");
+    if (!(flags & GPR_TRACE_DEVICE_PTRS) || (flags & GPR_TRACE_REF_ORDER))
+        return fail(GPR_ERR_INVALID, "gpr_trace_bvh4_stats: needs device rays and the BVH4 traversal");
+    unsigned long long h[3] = {0, 0, 0};
+    if (s->view().ntri > 0 && n > 0) {
+        unsigned long long* d = s->d_stats.as<unsigned long long>() + 8;
+        GPR_TRY(launch_trace_stats(s->view(), rays, n, (flags & GPR_TRACE_ANY) != 0, d, stream()));
+        GPR_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, stream()));
+        GPR_HIP(hipStreamSynchronize(stream()));
+    }
+    if (h[2]) return fail(GPR_ERR_INVALID, "traversal stack capacity exceeded on " + std::to_string(h[2]) + " rays");
+    if (node_fetches) *node_fetches = h[0];
+    if (tri_tests) *tri_tests = h[1];
+    return GPR_OK;
+}
+
+int gpr_trace_overflow_count(uint64_t* count) {
+    if (!count) return fail(GPR_ERR_INVALID, "gpr_trace_overflow_count: null argument");
+    unsigned long long ov = 0;
+    GPR_TRY(trace_overflow_count(&ov));
+    *count = ov;
+    return GPR_OK;
+}
+
+int gpr_release_stream(void* hip_stream) {
+    hipStream_t st = reinterpret_cast<hipStream_t>(hip_stream);
+    GPR_HIP(hipStreamSynchronize(st));
+    std::unique_ptr<TraceStaging> stage;
+    std::unique_ptr<SortScratch> sort;
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        auto it = g_stage.find({g_device, st});
+        if (it != g_stage.end()) {
+            // a host-buffer trace of another thread may still hold the set
+            std::lock_guard<std::mutex> busy(it->second->mu);
+            stage = std::move(it->second);
+            g_stage.erase(it);
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_sort_mu);
+        auto it = g_sort_scratch.find({g_device, st});
+        if (it != g_sort_scratch.end()) {
+            sort = std::move(it->second);
+            g_sort_scratch.erase(it);
+        }
+    }
+    return trace_release_stream(st);
+}
+
 // ------------------------------------------------------- generators ----
 int gpr_gen_soup(uint64_t seed, uint64_t stream_id, uint32_t n, float sigma, float* pos) {
     if (!pos && n) return fail(GPR_ERR_INVALID, "gpr_gen_soup: null output");

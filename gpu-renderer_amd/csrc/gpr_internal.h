@@ -208,6 +208,9 @@ int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64
                  unsigned long long* stats, hipStream_t st, int nt = 256, int sl = 16, int chunk = 64,
                  int refill = 0);
 bool trace_cfg_valid(int nt, int sl, int chunk);
+int launch_trace_stats(const SceneView& sv, const gpr_ray* rays, uint64_t n, bool any, unsigned long long* stats,
+                       hipStream_t st);
+int trace_release_stream(hipStream_t st);
 // imageio.cpp: ReadImage -- width*height*3 linear floats, bottom row first
 int read_image(const std::string& path, uint32_t* w, uint32_t* h, std::vector<float>* rgb);
 // Gauss-Jordan inverse of a row-major 4x4 (transform.cpp:30-90); false when

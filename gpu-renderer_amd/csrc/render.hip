@@ -982,6 +982,7 @@ struct gpr_renderer {
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t graph = nullptr;
     SceneView baked{};  // scene view (camera included) the graph was captured with
+    SceneView accum{};  // scene view the accumulated samples were taken with
     bool reset_pending = false;  // PathTracer::m_reset: the next render starts a new accumulation
     ~gpr_renderer() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -1194,7 +1195,17 @@ int gpr_renderer_reset(gpr_renderer* r) {
     }
     r->spp_done = 0;
     r->reset_pending = false;
+    r->accum = r->scene->view();
     return GPR_OK;
+}
+
+int gpr_renderer_set_opts(gpr_renderer* r, const gpr_render_opts* o) {
+    if (!r || !o) return fail(GPR_ERR_INVALID, "gpr_renderer_set_opts: null argument");
+    if (o->max_depth > kMaxBounce) return fail(GPR_ERR_INVALID, "max_depth above 64");
+    if (o->rng_mode > GPR_RNG_PCG || o->traversal > GPR_TRAVERSAL_REF)
+        return fail(GPR_ERR_INVALID, "gpr_renderer_set_opts: unknown mode");
+    r->opts = *o;  // baked into the sample graph: reset drops it
+    return gpr_renderer_reset(r);
 }
 
 int gpr_renderer_restore(gpr_renderer* r, const float* rgb, uint32_t spp) {
@@ -1211,6 +1222,7 @@ int gpr_renderer_restore(gpr_renderer* r, const float* rgb, uint32_t spp) {
     GPR_HIP(hipStreamSynchronize(stream()));  // tmp is released on return
     r->spp_done = spp;
     r->reset_pending = false;
+    r->accum = r->scene->view();
     return GPR_OK;
 }
 
@@ -1220,6 +1232,12 @@ int gpr_render(gpr_renderer* r, uint32_t spp) {
     if ((uint64_t)r->scene->width * r->scene->height != r->npix)
         return fail(GPR_ERR_INVALID, "gpr_render: the scene's image size changed; use gpr_renderer_resize");
     if (r->reset_pending) GPR_TRY(gpr_renderer_reset(r));
+    // The camera lives on the scene: another renderer's zoom / translate /
+    // rotate on the same scene, or a new camera set on it, changes the view
+    // under this renderer.  Samples of two views must not share one sum, so
+    // the accumulation starts again, as PathTracer::m_reset makes it
+    // (pathtracer.cu:177-200).
+    if (r->spp_done > 0 && !same_view(r->scene->view(), r->accum)) GPR_TRY(gpr_renderer_reset(r));
     hipStream_t st = stream();
     hipGraphExec_t g = nullptr;
     GPR_TRY(sample_graph(r, &g));

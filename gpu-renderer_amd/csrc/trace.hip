@@ -187,6 +187,40 @@ __global__ __launch_bounds__(256) void k_trace_ref(SceneView sv, const gpr_ray* 
     }
 }
 
+// Node fetches and triangle tests of the BVH4 walk, summed over a batch of
+// rays (the immediate-leaf walk of trav_step, which is what the closest-hit
+// kernels run for every ray).  Persistent grid-stride loop over the rays, so
+// the spill area of the (256, 16) configuration bounds the lanes.
+template <bool ANY>
+__global__ __launch_bounds__(256) void k_trace_stats(SceneView sv, const gpr_ray* __restrict__ rays, uint64_t n,
+                                                     unsigned long long* __restrict__ stats,
+                                                     int2* __restrict__ spill) {
+    __shared__ int2 lds[16 * 256];
+    LaneStackT<256, 16> st;
+    st.lds = lds + threadIdx.x;
+    st.spill = spill + (uint64_t)(blockIdx.x * 256 + threadIdx.x) * kStackGlobal;
+    st.sp = 0;
+    unsigned long long nf = 0, nt = 0, ov = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        Trav t;
+        int rc = trav_begin<ANY>(sv, load_ray(rays, i), st, t) ? kTravDone : kTravMore;
+        while (rc == kTravMore) rc = trav_step<ANY>(sv, st, t);
+        nf += t.fetched;
+        nt += t.tested + (sv.root4 < 0 ? 1u : 0u);
+        ov += rc == kTravOverflow ? 1u : 0u;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        nf += __shfl_down(nf, o, 64);
+        nt += __shfl_down(nt, o, 64);
+        ov += __shfl_down(ov, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(stats + 0, nf);
+        atomicAdd(stats + 1, nt);
+        atomicAdd(stats + 2, ov);
+    }
+}
+
 using TraceKernel = void (*)(SceneView, const gpr_ray*, gpr_hit*, uint64_t, unsigned long long*, int2*, uint32_t);
 struct TraceVariant {
     TraceKernel closest, any;          // wave-synchronous rays
@@ -299,6 +333,38 @@ int launch_trace(const SceneView& sv, const gpr_ray* rays, gpr_hit* hits, uint64
                        (uint32_t)chunk);
     GPR_HIP(hipGetLastError());
     return GPR_OK;
+}
+
+// BVH4 node fetches, triangle tests and overflows summed over n device rays
+// (stats: 3 device words, cleared here); the per-ray node sequence is the one
+// every fast closest-hit configuration walks.
+int launch_trace_stats(const SceneView& sv, const gpr_ray* rays, uint64_t n, bool any, unsigned long long* stats,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, stats, 3);
+    if (n == 0) return GPR_OK;
+    TraceResources* rp = nullptr;
+    GPR_TRY(resources_for(st, &rp));
+    const int k = trace_variant_index(256, 16);
+    const uint64_t want = (n + 255) / 256;
+    const dim3 grid((unsigned)std::min<uint64_t>(rp->grid[k][0], want));
+    if (any) hipLaunchKernelGGL(k_trace_stats<true>, grid, dim3(256), 0, st, sv, rays, n, stats, rp->spill.as<int2>());
+    else hipLaunchKernelGGL(k_trace_stats<false>, grid, dim3(256), 0, st, sv, rays, n, stats, rp->spill.as<int2>());
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+
+// Releases the fast-trace work buffers of (this thread's device, st) after
+// the stream's work has finished; the next trace on st allocates them again.
+int trace_release_stream(hipStream_t st) {
+    std::unique_ptr<TraceResources> victim;
+    {
+        std::lock_guard<std::mutex> lk(g_res_mu);
+        auto it = g_res.find({device_id(), st});
+        if (it == g_res.end()) return GPR_OK;
+        victim = std::move(it->second);
+        g_res.erase(it);
+    }
+    return GPR_OK;  // the buffers are freed as victim goes out of scope
 }
 
 // Overflow count of the last fast trace on this thread's device and stream

@@ -171,6 +171,7 @@ struct Trav {
     gpr_hit hit;
     int32_t node;
     uint32_t fetched;
+    uint32_t tested;  // triangle tests (read only by the statistics kernel; dead code elsewhere)
     bool found;
 };
 constexpr float kSlabCons = 1.0f + 2.0f * 1.7881394e-07f;  // 1 + 2*gamma(3)
@@ -184,6 +185,7 @@ GPR_D bool trav_begin(const SceneView& sv, const Ray3& r, Stack& st, Trav& t) {
     t.tmax = r.tmax;
     t.found = false;
     t.fetched = 0;
+    t.tested = 0;
     st.sp = 0;
     if (sv.root4 < 0) {
         tri_rec_hit(sv.tris, sv.ntri, ~sv.root4, r, t.tmax, t.hit, t.found);
@@ -238,6 +240,7 @@ GPR_D int trav_step(const SceneView& sv, Stack& st, Trav& t) {
         leaves &= leaves - 1;
         const int32_t leaf = c == 0 ? cha[0] : c == 1 ? cha[1] : c == 2 ? cha[2] : cha[3];
         tri_rec_hit(sv.tris, sv.ntri, ~leaf, r, t.tmax, t.hit, t.found);
+        ++t.tested;
         if (ANY && t.found) return kTravDone;
     }
     uint32_t key[4];

@@ -160,6 +160,10 @@ _SIG = {
                                  C.POINTER(C.c_uint32)]),
     "gpr_trace_set_cfg": (C.c_int, [_P, C.POINTER(TraceCfg)]),
     "gpr_trace_ref_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "gpr_trace_bvh4_stats": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64)]),
+    "gpr_trace_overflow_count": (C.c_int, [C.POINTER(C.c_uint64)]),
+    "gpr_release_stream": (C.c_int, [_P]),
     "gpr_renderer_create": (C.c_int, [_P, C.POINTER(RenderOpts), C.POINTER(_P)]),
     "gpr_render": (C.c_int, [_P, C.c_uint32]),
     "gpr_renderer_read": (C.c_int, [_P, _P, _P]),
@@ -168,6 +172,7 @@ _SIG = {
     "gpr_renderer_restore": (C.c_int, [_P, _P, C.c_uint32]),
     "gpr_renderer_resolve_rgba8": (C.c_int, [_P, _P, C.c_uint32]),
     "gpr_renderer_reset": (C.c_int, [_P]),
+    "gpr_renderer_set_opts": (C.c_int, [_P, C.POINTER(RenderOpts)]),
     "gpr_renderer_queue_sizes": (C.c_int, [_P, _P, _P, C.c_uint32]),
     "gpr_renderer_free": (C.c_int, [_P]),
     "gpr_scene_arena_bytes": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
@@ -461,6 +466,13 @@ class Scene:
         check(lib.gpr_trace_ref_stats(self.h, C.byref(nv), C.byref(nt)))
         return nv.value, nt.value
 
+    def bvh4_stats(self, rays_ptr, n, any_hit=False):
+        """(BVH4 node fetches, triangle tests) summed over n device rays."""
+        nf, nt = C.c_uint64(), C.c_uint64()
+        fl = TRACE_DEVICE_PTRS | (TRACE_ANY if any_hit else 0)
+        check(lib.gpr_trace_bvh4_stats(self.h, C.c_void_p(rays_ptr), n, fl, C.byref(nf), C.byref(nt)))
+        return nf.value, nt.value
+
     def free(self):
         if self.h:
             check(lib.gpr_scene_free(self.h))  # refused while renderers still use the scene
@@ -486,6 +498,11 @@ class Renderer:
 
     def render(self, spp):
         check(lib.gpr_render(self.h, spp))
+
+    def set_opts(self, max_depth=5, rng_mode=RNG_PCG, rr_start=3, seed=1, traversal=TRAVERSAL_FAST):
+        """New options for the next accumulation (resets it)."""
+        o = RenderOpts(max_depth, rng_mode, rr_start, seed, traversal)
+        check(lib.gpr_renderer_set_opts(self.h, C.byref(o)))
 
     def zoom(self, d):
         check(lib.gpr_renderer_zoom(self.h, d))
@@ -708,6 +725,19 @@ def sort_pairs_device_noalloc(keys_ptr, skeys_ptr, sidx_ptr, n):
     """gpr_sort_pairs_device with the library's fixed scratch array (no
     allocation; n at most 1,048,576)."""
     check(lib.gpr_sort_pairs_device_noalloc(C.c_void_p(keys_ptr), C.c_void_p(skeys_ptr), C.c_void_p(sidx_ptr), n))
+
+
+def trace_overflow_count():
+    """Rays that ran out of traversal stack in the last fast trace on this
+    thread's stream (device-pointer traces do not raise it themselves)."""
+    v = C.c_uint64()
+    check(lib.gpr_trace_overflow_count(C.byref(v)))
+    return v.value
+
+
+def release_stream(ptr):
+    """Free the trace / sort work buffers kept for this thread's device and stream."""
+    check(lib.gpr_release_stream(None if not ptr else C.c_void_p(ptr)))
 
 
 def set_stream(ptr):

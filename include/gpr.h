@@ -296,6 +296,26 @@ int gpr_trace_tune(gpr_scene* s, const gpr_ray* rays, uint64_t n, uint32_t flags
 int gpr_trace_set_cfg(gpr_scene* s, const gpr_trace_cfg* cfg);
 /* Node visits and triangle tests of the last GPR_TRACE_REF_ORDER call. */
 int gpr_trace_ref_stats(const gpr_scene* s, uint64_t* node_visits, uint64_t* tri_tests);
+/* BVH4 node fetches (128 B each) and triangle-record tests (48 B each) summed
+ * over n device rays (flags: GPR_TRACE_DEVICE_PTRS, optionally GPR_TRACE_ANY),
+ * for the walk the fast closest-hit kernels take (any hit: with leaves tested
+ * as soon as their node is visited).  No hits are written.  No counterpart in
+ * the reference (its traversal keeps no counters, pathtracer_kernel.h:175-321). */
+int gpr_trace_bvh4_stats(gpr_scene* s, const gpr_ray* rays, uint64_t n, uint32_t flags, uint64_t* node_fetches,
+                         uint64_t* tri_tests);
+/* Rays whose traversal ran out of stack in the last fast gpr_trace on the
+ * calling thread's (device, stream); 0 = every ray completed.  Host-buffer
+ * calls report overflow as an error themselves; device-pointer calls return
+ * without waiting, so their callers read it here (this call synchronizes the
+ * stream).  The reference's fixed int[64] stack has no check at all
+ * (pathtracer_kernel.h:183-233). */
+int gpr_trace_overflow_count(uint64_t* count);
+/* Frees the work buffers that gpr_trace, gpr_sort_pairs_device and
+ * host-buffer traces keep for the calling thread's device and hip_stream
+ * (NULL = null stream), after synchronizing that stream; the next call on the
+ * stream allocates them again.  For applications that create streams per
+ * request or per thread.  No counterpart in the reference (one stream). */
+int gpr_release_stream(void* hip_stream);
 
 /* ----------------------------------------------------------- renderer --- */
 #define GPR_RNG_REFERENCE 0u /* TEA seed + LCG, slot-indexed seeds (pathtracer_kernel.h:92, rayqueue.h) */
@@ -340,6 +360,11 @@ int gpr_renderer_write_image(gpr_renderer* r, const char* path);
 int gpr_renderer_resolve_rgba8(gpr_renderer* r, uint32_t* out, uint32_t flags);
 /* PathTracer::reset (pathtracer.cu:195-200). */
 int gpr_renderer_reset(gpr_renderer* r);
+/* New options (seed, depth, roulette start, RNG mode, traversal) for the next
+ * accumulation; resets it.  The reference fixes PTOptions at construction
+ * (pathtracer.h:7-14, pathtracer.cu:8-25): this lets one renderer serve
+ * requests that differ only in their options. */
+int gpr_renderer_set_opts(gpr_renderer* r, const gpr_render_opts* opts);
 /* Interactive camera and frame operations of PathTracer (pathtracer.cu:177-209):
  * they change the scene's free camera (Camera::zoom / translate / rotate,
  * camera.h:40-63) or the image size, and the next gpr_render starts a new

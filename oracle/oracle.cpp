@@ -1105,15 +1105,54 @@ struct Ctx {
     const uint32_t* lights;
     uint32_t nlights;
     Cam cam;
+    uint8_t* ties;  // per pixel: a closest hit of its paths had a near tie (may be null)
 };
 
-inline orc_hit trace_ray(const Ctx& c, f3 o, f3 d, float tmin, float tmax, bool any) {
+// Whether a triangle other than hit.id meets the ray at a t within 1e-6
+// relative of hit.t.  Such a closest hit is decided by the visit order (the
+// reference keeps the last triangle accepted, pathtracer_kernel.h:175-233), so
+// a traversal in another order may pick the other triangle.
+bool near_tie(const orc_node* nodes, const float* pos, const orc_ray& r, const orc_hit& hit) {
+    if (hit.id < 0) return false;
+    const double lo = (double)hit.t - 1e-6 * (double)hit.t;
+    const float hi = (float)((double)hit.t + 1e-6 * (double)hit.t);
+    f3 o = ld3(r.o), d = ld3(r.d);
+    f3 inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+    int neg[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
+    int cur = 0, sp = 0;
+    int stack[64];
+    while (true) {
+        const orc_node& nd = nodes[cur];
+        if (aabb_ref(nd, o, r.tmin, hi, inv, neg)) {
+            if (nd.leaf & 0xff) {
+                const float* p = pos + 9ull * nd.idx;
+                float t, u, v;
+                if ((int32_t)nd.idx != hit.id &&
+                    intersect_ref(ld3(p), ld3(p + 3), ld3(p + 6), o, d, r.tmin, hi, &t, &u, &v) && (double)t >= lo)
+                    return true;
+                if (sp == 0) break;
+                cur = stack[--sp];
+            } else {
+                stack[sp++] = cur + 1;
+                cur = (int)nd.idx;
+            }
+        } else {
+            if (sp == 0) break;
+            cur = stack[--sp];
+        }
+    }
+    return false;
+}
+
+inline orc_hit trace_ray(const Ctx& c, f3 o, f3 d, float tmin, float tmax, bool any, uint32_t pix = ~0u) {
     orc_ray r;
     r.o[0] = o.x; r.o[1] = o.y; r.o[2] = o.z;
     r.d[0] = d.x; r.d[1] = d.y; r.d[2] = d.z;
     r.tmin = tmin;
     r.tmax = tmax;
-    return trace_one(c.nodes, c.sc->pos, r, any, nullptr, nullptr);
+    orc_hit h = trace_one(c.nodes, c.sc->pos, r, any, nullptr, nullptr);
+    if (c.ties && !any && pix != ~0u && near_tie(c.nodes, c.sc->pos, r, h)) c.ties[pix] = 1;
+    return h;
 }
 
 // Queue entry of the reference's RayQueue (rayqueue.h:6-47).
@@ -1267,7 +1306,7 @@ void render_path_pcg(const Ctx& c, const orc_render_opts* op, uint32_t pix, uint
     e.spec = false;
     e.pdf = 0.f;
     e.pix = pix;
-    e.hit = trace_ray(c, e.o, e.d, e.tmin, e.tmax, false);
+    e.hit = trace_ray(c, e.o, e.d, e.tmin, e.tmax, false, pix);
     for (uint32_t bounce = 0; bounce < op->max_depth; ++bounce) {
         float smp[8];
         for (int i = 0; i < 8; ++i) smp[i] = pcg_float(st, inc);
@@ -1279,7 +1318,7 @@ void render_path_pcg(const Ctx& c, const orc_render_opts* op, uint32_t pix, uint
               [&](const QE& s) { scat = s; have_scatter = true; });
         if (have_shadow) add_shadow(c, sh, fb);
         if (!have_scatter) break;
-        scat.hit = trace_ray(c, scat.o, scat.d, scat.tmin, scat.tmax, false);
+        scat.hit = trace_ray(c, scat.o, scat.d, scat.tmin, scat.tmax, false, pix);
         bsdf_light_contrib(c, scat, fb);
         st = scat.rng;
         e = scat;
@@ -1596,7 +1635,13 @@ void orc_ggx_sample_wh(const float wo[3], const float* u, uint32_t n, float ax, 
 
 int orc_render(const orc_scene* scene, const orc_node* nodes, const orc_render_opts* opts,
                float* rgb_sum, uint32_t* sample_num, int nthreads) {
+    return orc_render_ties(scene, nodes, opts, rgb_sum, sample_num, nullptr, nthreads);
+}
+
+int orc_render_ties(const orc_scene* scene, const orc_node* nodes, const orc_render_opts* opts,
+                    float* rgb_sum, uint32_t* sample_num, uint8_t* tie_pixels, int nthreads) {
     if (!scene || !opts || scene->width == 0 || scene->height == 0) return -1;
+    if (tie_pixels && opts->rng_mode == 0) return -1;  // pcg mode only
     std::vector<uint32_t> lights;
     for (uint32_t i = 0; i < scene->ntri; ++i)
         if (is_light(scene, i)) lights.push_back(i);
@@ -1606,6 +1651,7 @@ int orc_render(const orc_scene* scene, const orc_node* nodes, const orc_render_o
     c.lights = lights.data();
     c.nlights = (uint32_t)lights.size();
     c.cam = make_camera(scene);
+    c.ties = tie_pixels;
     uint32_t npix = scene->width * scene->height;
     if (scene->ntri == 0) {
         // nothing to hit: only the environment term (a constant one)

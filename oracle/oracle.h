@@ -165,6 +165,12 @@ void orc_ggx_sample_wh(const float wo[3], const float* u, uint32_t n, float ax, 
 int orc_render(const orc_scene* scene, const orc_node* nodes,
                const orc_render_opts* opts, float* rgb_sum, uint32_t* sample_num,
                int nthreads);
+/* orc_render in pcg mode that also sets tie_pixels[p] = 1 when a closest hit on
+ * one of pixel p's paths had another triangle within 1e-6 relative of its t
+ * (a hit whose triangle depends on the traversal's visit order). */
+int orc_render_ties(const orc_scene* scene, const orc_node* nodes,
+                    const orc_render_opts* opts, float* rgb_sum, uint32_t* sample_num,
+                    uint8_t* tie_pixels, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -5,7 +5,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+# GPR_ORACLE_LIB names another build of the same source (the benchmark's CPU
+# baseline compiles one with -O3 -march=native for the host it runs on)
+PATH = os.environ.get("GPR_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("bmax", "<f4", 3), ("idx", "<u4"), ("leaf", "<u4")])
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("tmin", "<f4"), ("tmax", "<f4")])
@@ -98,6 +100,8 @@ lib.orc_env_le.restype = None
 lib.orc_env_le.argtypes = [C.POINTER(Scene), P, P, P]
 lib.orc_render.restype = C.c_int
 lib.orc_render.argtypes = [C.POINTER(Scene), P, C.POINTER(RenderOpts), P, P, C.c_int]
+lib.orc_render_ties.restype = C.c_int
+lib.orc_render_ties.argtypes = [C.POINTER(Scene), P, C.POINTER(RenderOpts), P, P, P, C.c_int]
 
 NTHREADS = min(8, os.cpu_count() or 1)
 
@@ -204,3 +208,16 @@ def render(scene, nodes, spp, max_depth, rng_mode, seed=1, rr_start=3, first_sam
     rc = lib.orc_render(C.byref(scene), ptr(nodes), C.byref(o), ptr(rgb), ptr(cnt), nthreads)
     assert rc == 0
     return rgb, cnt
+
+
+def render_ties(scene, nodes, spp, max_depth, seed=1, rr_start=3, first_sample=0, nthreads=NTHREADS):
+    """pcg-mode render plus a per-pixel flag: some closest hit on the pixel's
+    paths had another triangle within 1e-6 relative of its t."""
+    n = scene.width * scene.height
+    rgb = np.zeros((n, 3), np.float32)
+    cnt = np.zeros(n, np.uint32)
+    ties = np.zeros(n, np.uint8)
+    o = RenderOpts(spp, max_depth, 1, rr_start, seed, first_sample)
+    rc = lib.orc_render_ties(C.byref(scene), ptr(nodes), C.byref(o), ptr(rgb), ptr(cnt), ptr(ties), nthreads)
+    assert rc == 0
+    return rgb, cnt, ties.astype(bool)

@@ -5,7 +5,9 @@ With the reference visit order on the GPU the float work is the same, so the
 frame buffers must agree bit for bit in both RNG modes (reference TEA/LCG with
 slot-indexed seeds, and PCG32 carried with the path).  With the fast BVH4
 traversal only ties of t may differ: per-pixel relative error <= 1e-4 on
-every pixel but a tiny tail, which must stay within a loose bound.
+every pixel whose paths met no near tie (another triangle within 1e-6
+relative of a closest hit's t, where the triangle picked depends on the visit
+order), as the oracle reports per pixel.
 """
 import numpy as np
 import pytest
@@ -24,6 +26,26 @@ def oracle_render(scene, spp, depth, rng_mode, seed, rr_start=3):
     nodes, _, _ = O.lbvh(f["pos"])
     rgb, cnt = O.render(osc, nodes, spp, depth, rng_mode, seed=seed, rr_start=rr_start)
     return rgb, cnt
+
+
+def oracle_render_ties(scene, spp, depth, seed, rr_start=3, nthreads=None):
+    f = scene.flat()
+    info = f["info"]
+    osc = O.make_scene(f, np.array(info.c2w[:], np.float32), info.fov_deg, info.width, info.height,
+                       env=(list(info.env) if info.has_env else None), info=info)
+    nodes, _, _ = O.lbvh(f["pos"])
+    kw = {} if nthreads is None else {"nthreads": nthreads}
+    return O.render_ties(osc, nodes, spp, depth, seed=seed, rr_start=rr_start, **kw)
+
+
+def check_fast_tail(g, o, ties, label):
+    """Every pixel outside 1e-4 relative must have met a near tie on its paths."""
+    ok = rel_close(g, o)
+    unexplained = np.nonzero(~ok & ~ties)[0]
+    print(f"[render] {label}: {(~ok).sum()} of {len(ok)} pixels outside 1e-4, "
+          f"{ties.sum()} pixels with a near tie, {len(unexplained)} unexplained", flush=True)
+    assert len(unexplained) == 0, (label, unexplained[:10], g[unexplained[:3]], o[unexplained[:3]])
+    return ok
 
 
 def gpu_render(scene, spp, depth, rng_mode, seed, traversal, rr_start=3):
@@ -65,11 +87,8 @@ def test_cornell_c1_fast_traversal_pcg():
     # the C1 configuration: 256x256, 16 spp, depth 5, PCG32 seed 1
     s = gpr.Scene.cornell(256, 256)
     g, _, _, _ = gpu_render(s, 16, 5, gpr.RNG_PCG, 1, gpr.TRAVERSAL_FAST)
-    o, _ = oracle_render(s, 16, 5, gpr.RNG_PCG, 1)
-    ok = rel_close(g, o)
-    frac = ok.mean()
-    print(f"[render] C1 per-pixel 1e-4 agreement {frac:.6f} ({(~ok).sum()} pixels differ)", flush=True)
-    assert frac >= 0.999
+    o, _, ties = oracle_render_ties(s, 16, 5, 1)
+    check_fast_tail(g, o, ties, "C1 fast traversal")
     # the mean image agrees far more tightly than a single path flip
     assert abs(g.mean() - o.mean()) <= 1e-3 * o.mean()
 
@@ -87,10 +106,8 @@ def test_cornell_fast_traversal_reference_rng_statistics():
 def test_sphere_grid_fast_traversal():
     s = gpr.Scene.sphere_grid(96, 54, 16)
     g, _, _, _ = gpu_render(s, 4, 8, gpr.RNG_PCG, 7, gpr.TRAVERSAL_FAST)
-    o, _ = oracle_render(s, 4, 8, gpr.RNG_PCG, 7)
-    frac = rel_close(g, o).mean()
-    print(f"[render] sphere grid fast traversal agreement {frac:.6f}", flush=True)
-    assert frac >= 0.995
+    o, _, ties = oracle_render_ties(s, 4, 8, 7)
+    check_fast_tail(g, o, ties, "sphere grid fast traversal")
 
 
 def test_sphere_grid_full_size_one_sample():
@@ -108,8 +125,7 @@ def test_sphere_grid_full_size_one_sample():
                        env=(list(info.env) if info.has_env else None), info=info)
     nodes, _, _ = O.lbvh(f["pos"])
     t0 = time.time()
-    o, ocnt = O.render(osc, nodes, 1, 8, gpr.RNG_PCG, seed=7, rr_start=3,
-                       nthreads=min(16, os.cpu_count() or 1))
+    o, ocnt, ties = O.render_ties(osc, nodes, 1, 8, seed=7, rr_start=3, nthreads=min(16, os.cpu_count() or 1))
     lit = float((o.sum(axis=1) > 0).mean())
     print(f"[render] full-size oracle sample {time.time() - t0:.1f} s, lit pixels {lit:.3f}", flush=True)
     assert np.array_equal(ocnt, np.ones_like(ocnt)) and lit > 0.1
@@ -118,9 +134,7 @@ def test_sphere_grid_full_size_one_sample():
     bad = np.nonzero(np.any(g.view(np.uint32) != o.view(np.uint32), axis=1))[0]
     assert len(bad) == 0, (len(bad), bad[:10], g[bad[:3]], o[bad[:3]])
     gf, _, _, _ = gpu_render(s, 1, 8, gpr.RNG_PCG, 7, gpr.TRAVERSAL_FAST)
-    frac = rel_close(gf, o).mean()
-    print(f"[render] full-size fast traversal agreement {frac:.6f}", flush=True)
-    assert frac >= 0.995
+    check_fast_tail(gf, o, ties, "full-size fast traversal")
     assert abs(float(gf.mean()) - float(o.mean())) <= 1e-3 * float(o.mean())
 
 
@@ -299,3 +313,44 @@ def test_scene_free_refused_while_renderers_live():
         r.free()
     s.free()
     assert s.h is None
+
+
+def test_shared_scene_camera_change_restarts_accumulation():
+    # two renderers on one scene with the reference's free (GUI) camera: the
+    # camera lives on the scene, so rotating one renderer moves the other's
+    # view too; the other's next render must not add samples of the new view
+    # to the sum of the old one
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cbox", "cbox_mesh.xml")
+    s = gpr.Scene.load(path)
+    a = gpr.Renderer(s, max_depth=4, rng_mode=gpr.RNG_PCG, seed=5)
+    a.resize(40, 32)
+    b = gpr.Renderer(s, max_depth=4, rng_mode=gpr.RNG_PCG, seed=5)
+    b.render(2)
+    a.rotate(0.05, -0.02)
+    a.render(1)
+    b.render(3)  # new view: a fresh accumulation of 3 samples
+    got, cnt = b.read()
+    fresh = gpr.Renderer(s, max_depth=4, rng_mode=gpr.RNG_PCG, seed=5)
+    fresh.render(3)
+    want, wcnt = fresh.read()
+    assert np.all(cnt == 3) and np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # a camera set on the scene directly restarts it as well
+    c2w = gpr.lookat((0, 0, 0), (0.2, 0.1, 3.9))
+    s.set_camera(c2w, 40.0, 40, 32)
+    b.render(2)
+    got2, cnt2 = b.read()
+    fresh.reset()
+    fresh.render(2)
+    want2, _ = fresh.read()
+    assert np.all(cnt2 == 2) and np.array_equal(got2.view(np.uint32), want2.view(np.uint32))
+    # new options on an existing renderer = a new renderer with those options
+    b.set_opts(max_depth=6, rng_mode=gpr.RNG_PCG, rr_start=2, seed=77)
+    b.render(2)
+    got3, _ = b.read()
+    c = gpr.Renderer(s, max_depth=6, rng_mode=gpr.RNG_PCG, rr_start=2, seed=77)
+    c.render(2)
+    want3, _ = c.read()
+    assert np.array_equal(got3.view(np.uint32), want3.view(np.uint32))
+    for r in (a, b, c, fresh):
+        r.free()

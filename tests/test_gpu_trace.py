@@ -166,6 +166,24 @@ def test_fast_vs_fp64_bruteforce_1m():
     ref, onv, ont = O.trace(onodes, pos, rays[:65536])
     assert np.array_equal(href[:65536].view(np.uint32), ref.view(np.uint32))
     compare_fast(h[:65536], ref, pos, rays[:65536], "closest 1M")
+    # the second pass of the workload: any-hit shadow rays to random in-cube
+    # targets (t in [0, 0.9999]) on all 4M rays; reference order bit-exact and
+    # the fast traversal's occlusion flag exact on a 65,536-ray slice, and the
+    # occlusion flag equal to a brute-force intersector on a sample
+    sh = gpr.gen_shadow_rays(2, 2, rays)
+    print("[trace] 1M any-hit shadow pass", flush=True)
+    a_ref_gpu = s.trace(sh, any_hit=True, ref_order=True)
+    a_fast = s.trace(sh, any_hit=True)
+    aref, _, _ = O.trace(onodes, pos, sh[:65536], any_hit=True)
+    assert np.array_equal(a_ref_gpu[:65536].view(np.uint32), aref.view(np.uint32))
+    assert np.array_equal(a_fast["id"][:65536] != -1, aref["id"] != -1)
+    occ = a_fast["id"] != -1
+    # a dense soup: almost every segment is blocked (99.8 %), but not all
+    assert occ.mean() > 0.5 and (~occ).sum() > 1000
+    hit = a_fast[occ]
+    assert np.all((hit["t"] >= 0) & (hit["t"] <= np.float32(0.9999)))
+    bfa, _ = O.brute(pos, sh[sel], any_hit=True)
+    assert np.array_equal(a_fast["id"][sel] != -1, bfa["id"] != -1)
 
 
 def test_axis_parallel_and_degenerate_rays():
@@ -223,3 +241,40 @@ def test_launch_configurations_give_identical_hits():
     s.trace_device(d_rays.data_ptr(), d_hits.data_ptr(), len(rays))
     torch.cuda.synchronize()
     assert np.array_equal(d_hits.cpu().numpy(), ref)
+
+
+def test_bvh4_stats_overflow_count_and_stream_release():
+    import torch
+    pos = G.soup(1 << 16)
+    s = G.scene_from_soup(pos)
+    n = 1 << 16
+    rays = rays_for(n)
+    d_r = torch.from_numpy(rays.view(np.uint8)).cuda()
+    d_h = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
+    nf, nt = s.bvh4_stats(d_r.data_ptr(), n)
+    af, at = s.bvh4_stats(d_r.data_ptr(), n, any_hit=True)
+    nodes4 = (len(pos) + 2) // 3  # BVH4 nodes are at least ntri / 3
+    print(f"[trace] 64K soup: {nf / n:.2f} BVH4 nodes, {nt / n:.2f} triangle tests per ray", flush=True)
+    assert n <= nf < n * nodes4 and 0 < nt < n * len(pos)
+    assert af <= nf  # any hit stops at its first occluder on the same walk
+    # one triangle: the root is a leaf, every ray tests it once and fetches no node
+    one = G.scene_from_soup(G.soup(1))
+    assert one.bvh4_stats(d_r.data_ptr(), n) == (0, n)
+    s.trace_device(d_r.data_ptr(), d_h.data_ptr(), n)
+    assert gpr.trace_overflow_count() == 0
+    h0 = d_h.cpu().numpy().copy()
+    # release this stream's work buffers: the next trace allocates them again
+    st = torch.cuda.Stream()
+    try:
+        gpr.set_stream(st.cuda_stream)
+        s.trace_device(d_r.data_ptr(), d_h.data_ptr(), n)
+        gpr.release_stream(st.cuda_stream)
+        gpr.release_stream(st.cuda_stream)  # nothing left to release: no error
+        d_h.zero_()
+        torch.cuda.synchronize()
+        s.trace_device(d_r.data_ptr(), d_h.data_ptr(), n)
+        st.synchronize()
+        assert np.array_equal(d_h.cpu().numpy(), h0)
+        assert gpr.trace_overflow_count() == 0
+    finally:
+        gpr.set_stream(0)
