@@ -195,13 +195,18 @@ __global__ __launch_bounds__(kB) void k_weights(const uint32_t* __restrict__ sk,
 // right child).  The second child to arrive at a parent forms its box.
 //
 // Everything one thread publishes for another (boxes, child ids, range
-// bounds) is written with device-scope relaxed atomic stores, which write
-// through to the device-coherent level; the writer waits for their completion
-// before it bumps the arrival counter (a device-scope RMW) and the reader
-// loads with device-scope atomic loads only after seeing that counter.  No L2
-// write-back fence is needed although the eight XCD L2 caches are not
-// coherent with each other.  Values read back are range-checked, so a
-// protocol failure raises the error word instead of indexing out of bounds.
+// bounds) is written with device-scope relaxed atomic stores, which gfx950
+// issues as write-through stores (sc1) to the device-coherent level; the
+// writer waits for their completion (vmcnt(0)) before it bumps the arrival
+// counter (a device-scope RMW, performed at the coherent level), and the
+// reader loads with device-scope atomic loads (sc1, served from the coherent
+// level) only after that RMW has returned.  So no L2 write-back (buffer_wbl2)
+// or invalidate is needed although the eight XCD L2 caches are not coherent
+// with each other; device-scope release / acquire on the RMW would add both
+// on every arrival (measured 16x slower at 1M triangles).  Values read back
+// are range-checked, and k_export_ref checks every parent box against the
+// union of its children's, so a protocol failure raises the error word
+// instead of indexing out of bounds or producing a wrong tree silently.
 GPR_D void pub(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 GPR_D uint32_t get(uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 GPR_D void pub2(uint64_t* p, float a, float b) {
@@ -253,9 +258,15 @@ __global__ __launch_bounds__(kB) void k_tree(const float* __restrict__ pos, uint
             pub(&child[2 * p + 1], node);
             pub(&rhi[p], r);
         }
-        __builtin_amdgcn_s_waitcnt(0);  // every publish above has completed
+        // every publish above has completed (s_waitcnt vmcnt(0)) before the
+        // arrival RMW is issued; the fence keeps the compiler from moving the
+        // publishing stores past the wait
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
         if (__hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
-        // second arrival: the sibling's publishes completed before its increment
+        // second arrival: the sibling's publishes completed before its
+        // increment; the loads below are issued after this RMW returned
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         if (as_left) r = get(&rhi[p]);
         else l = get(&rlo[p]);
         const uint32_t sib = get(&child[2 * p + (as_left ? 1 : 0)]);
@@ -307,9 +318,35 @@ __global__ __launch_bounds__(kB) void k_export_ref(uint32_t n, const uint32_t* _
                                                    const uint32_t* __restrict__ sidx,
                                                    const float* __restrict__ box,
                                                    const uint32_t* __restrict__ pre,
-                                                   gpr_ref_node* __restrict__ ref) {
+                                                   gpr_ref_node* __restrict__ ref,
+                                                   uint32_t* __restrict__ err) {
     const uint32_t x = blockIdx.x * kB + threadIdx.x;
-    if (x >= 2 * n - 1 || pre[x] >= 2 * n - 1) return;
+    if (x >= 2 * n - 1) return;
+    if (pre[x] >= 2 * n - 1) {
+        atomicOr(err, 4u);  // a node the walk from the root did not reach
+        return;
+    }
+    if (x < n - 1) {
+        // the parent box must be the union of its children's boxes as they
+        // stand now (a stale box read in k_tree would break the equality)
+        const uint32_t c0 = child[2 * x], c1 = child[2 * x + 1];
+        if (c0 >= 2 * n - 1 || c1 >= 2 * n - 1) {
+            atomicOr(err, 2u);
+        } else {
+            const float* a = box + 6ull * c0;
+            const float* b = box + 6ull * c1;
+            const float* p = box + 6ull * x;
+            const v3 lo = vmin3(V(a[0], a[1], a[2]), V(b[0], b[1], b[2]));
+            const v3 hi = vmax3(V(a[3], a[4], a[5]), V(b[3], b[4], b[5]));
+            const bool same = __float_as_uint(lo.x) == __float_as_uint(p[0]) &&
+                              __float_as_uint(lo.y) == __float_as_uint(p[1]) &&
+                              __float_as_uint(lo.z) == __float_as_uint(p[2]) &&
+                              __float_as_uint(hi.x) == __float_as_uint(p[3]) &&
+                              __float_as_uint(hi.y) == __float_as_uint(p[4]) &&
+                              __float_as_uint(hi.z) == __float_as_uint(p[5]);
+            if (!same) atomicOr(err, 2u);
+        }
+    }
     gpr_ref_node o;
     const float* b = box + 6ull * x;
     o.bmin[0] = b[0]; o.bmin[1] = b[1]; o.bmin[2] = b[2];
@@ -518,7 +555,8 @@ int build_bvh(gpr_scene* s) {
                            child, rlo, rhi, arrive, box, root);
         GPR_HIP(hipEventRecord(ev[3], st));
         hipLaunchKernelGGL(k_walkup, dim3(div_up(nn, kB)), dim3(kB), 0, st, n, parent, child, rlo, root, pre, depth);
-        hipLaunchKernelGGL(k_export_ref, dim3(div_up(nn, kB)), dim3(kB), 0, st, n, child, sidx, box, pre, ref);
+        hipLaunchKernelGGL(k_export_ref, dim3(div_up(nn, kB)), dim3(kB), 0, st, n, child, sidx, box, pre, ref,
+                           root + 1);
         hipLaunchKernelGGL(k_bvh4_flags, dim3(div_up(nn, kB)), dim3(kB), 0, st, n, pre, depth, flag);
         hipLaunchKernelGGL(k_scan_blocks, dim3(nscan_blocks), dim3(kB), 0, st, flag, (uint64_t)nn, id4, bsum);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kB), 0, st, bsum, nscan_blocks, total4);
@@ -533,7 +571,11 @@ int build_bvh(gpr_scene* s) {
         GPR_HIP(hipMemcpyAsync(&nb4, total4, 4, hipMemcpyDeviceToHost, st));
         GPR_HIP(hipMemcpyAsync(&tree_err, root + 1, 4, hipMemcpyDeviceToHost, st));
         GPR_HIP(hipStreamSynchronize(st));
-        if (tree_err) return fail(GPR_ERR_INVALID, "LBVH build: inconsistent node publication");
+        if (tree_err)
+            return fail(GPR_ERR_INVALID, std::string("LBVH build: ") +
+                                             ((tree_err & 1u) ? "inconsistent node publication " : "") +
+                                             ((tree_err & 2u) ? "parent box differs from the union of its children " : "") +
+                                             ((tree_err & 4u) ? "node unreachable from the root" : ""));
         B.nbvh4 = nb4;
         if (n > 1 && (nb4 == 0 || nb4 > n - 1)) return fail(GPR_ERR_INVALID, "LBVH build produced an invalid BVH4 node count");
         B.root4 = n == 1 ? ~(int32_t)0 : 0;
