@@ -262,6 +262,171 @@ GPR_D uint32_t lower_bound_u64(const unsigned long long* a, uint32_t n, unsigned
     return lo;
 }
 
+// Sorts x[0, n) (composites) into skeys / sidx[0, n): LDS bitonic sorts of
+// 2048-entry chunks, then pairwise merges through y[0, n) (global scratch).
+// The general fallback: any size, any key distribution.
+GPR_D void sort_range_merge(unsigned long long* x, unsigned long long* y, uint32_t n, unsigned long long* buf,
+                            uint32_t* __restrict__ skeys, uint32_t* __restrict__ sidx) {
+    constexpr uint32_t kChunk = 2048;
+    for (uint32_t c0 = 0; c0 < n; c0 += kChunk) {
+        const uint32_t m = min(kChunk, n - c0);
+        for (uint32_t i = threadIdx.x; i < kChunk; i += kLocalThreads) buf[i] = i < m ? x[c0 + i] : ~0ull;
+        __syncthreads();
+        bitonic_lds(buf, kChunk);
+        for (uint32_t i = threadIdx.x; i < m; i += kLocalThreads) x[c0 + i] = buf[i];
+        __syncthreads();
+    }
+    for (uint32_t w = kChunk; w < n; w <<= 1) {
+        for (uint32_t e = threadIdx.x; e < n; e += kLocalThreads) {
+            const uint32_t r0 = (e / (2 * w)) * (2 * w);
+            const uint32_t mid = min(r0 + w, n), end = min(r0 + 2 * w, n);
+            const unsigned long long v = x[e];
+            const uint32_t o = e < mid ? (e - r0) + lower_bound_u64(x + mid, end - mid, v)
+                                       : (e - mid) + lower_bound_u64(x + r0, mid - r0, v);
+            y[r0 + o] = v;
+        }
+        __syncthreads();
+        unsigned long long* t = x;
+        x = y;
+        y = t;
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += kLocalThreads) {
+        const unsigned long long v = x[i];
+        skeys[i] = (uint32_t)(v >> 32);
+        sidx[i] = (uint32_t)v;
+    }
+    __syncthreads();
+}
+
+// First sub-bucket start >= pos (starts are non-decreasing; an empty
+// sub-bucket starts where the next one does), or n.
+GPR_D uint32_t first_start_at(const uint32_t* start, uint32_t nsub, uint32_t n, uint32_t pos) {
+    uint32_t lo = 0, hi = nsub;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (start[mid] < pos) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < nsub ? start[lo] : n;
+}
+
+constexpr uint32_t kWinStep = 1536;   // window starts fall on sub-bucket starts past multiples of this
+constexpr uint32_t kRankMax = 1024;   // largest sub-bucket placed by rank (O(size) LDS reads per entry)
+static_assert(kWinStep + kRankMax <= 2 * kFastCap, "a window must fit the LDS buffer");
+
+// A bucket of n composites at src (any order, any size): split by the
+// sub-bucket digit (key bits below the top digit) into dst = the bucket's
+// range of the scratch, then cut the split into windows of whole
+// consecutive sub-buckets (each starts at the first sub-bucket start at or
+// past a multiple of kWinStep, so it holds at most kWinStep + kRankMax - 1
+// entries when no sub-bucket exceeds kRankMax) and place every entry of a
+// window by its rank among its sub-bucket, all in LDS.  A window with a
+// sub-bucket above kRankMax is sorted by sort_range_merge, with src (free
+// after the split) as its scratch.
+GPR_D void large_bucket(unsigned long long* src, unsigned long long* dst, uint32_t n, uint32_t sub_shift,
+                        uint32_t nsub, uint32_t* __restrict__ skeys, uint32_t* __restrict__ sidx,
+                        unsigned long long* buf, uint32_t* cnt, uint32_t* start, uint32_t* sh, uint32_t* flag) {
+    // global reads in batches of kBatch per thread, all in flight before their first use
+    constexpr uint32_t kBatch = 8;
+    for (uint32_t s = threadIdx.x; s < kSub; s += kLocalThreads) cnt[s] = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += kBatch * kLocalThreads) {
+        unsigned long long v[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const uint32_t i = base + j * kLocalThreads + threadIdx.x;
+            v[j] = i < n ? src[i] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j)
+            if (base + j * kLocalThreads + threadIdx.x < n) atomicAdd(&cnt[(uint32_t)(v[j] >> sub_shift) & (nsub - 1)], 1u);
+    }
+    __syncthreads();
+    constexpr int kQ = kSub / kLocalThreads;
+    uint32_t cq[kQ], sq = 0;
+#pragma unroll
+    for (int k = 0; k < kQ; ++k) {
+        cq[k] = cnt[kQ * threadIdx.x + k];
+        sq += cq[k];
+    }
+    uint32_t tot;
+    uint32_t run = scan_incl<kLocalThreads>(sq, sh, &tot) - sq;
+    uint32_t* fill = reinterpret_cast<uint32_t*>(buf);  // buf is free during the split
+#pragma unroll
+    for (int k = 0; k < kQ; ++k) {
+        start[kQ * threadIdx.x + k] = run;
+        fill[kQ * threadIdx.x + k] = 0;
+        run += cq[k];
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += kBatch * kLocalThreads) {
+        unsigned long long v[kBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            const uint32_t i = base + j * kLocalThreads + threadIdx.x;
+            v[j] = i < n ? src[i] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kBatch; ++j) {
+            if (base + j * kLocalThreads + threadIdx.x < n) {
+                const uint32_t sb = (uint32_t)(v[j] >> sub_shift) & (nsub - 1);
+                dst[start[sb] + atomicAdd(&fill[sb], 1u)] = v[j];
+            }
+        }
+    }
+    __syncthreads();  // the split is in dst, seen by every thread of the block
+    const uint32_t nw = (n + kWinStep - 1) / kWinStep;
+    for (uint32_t w = 0; w < nw; ++w) {
+        const uint32_t lo = first_start_at(start, nsub, n, w * kWinStep);
+        const uint32_t hi = first_start_at(start, nsub, n, (w + 1) * kWinStep);
+        const uint32_t m = hi - lo;
+        if (m == 0) continue;
+        if (threadIdx.x == 0) *flag = m > 2u * kFastCap ? 1u : 0u;
+        __syncthreads();
+        if (*flag == 0) {
+            constexpr uint32_t kW = (2 * kFastCap + kLocalThreads - 1) / kLocalThreads;
+            unsigned long long v[kW];
+#pragma unroll
+            for (uint32_t j = 0; j < kW; ++j) {
+                const uint32_t i = j * kLocalThreads + threadIdx.x;
+                v[j] = i < m ? dst[lo + i] : 0ull;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kW; ++j) {
+                const uint32_t i = j * kLocalThreads + threadIdx.x;
+                if (i < m) {
+                    buf[i] = v[j];
+                    if (cnt[(uint32_t)(v[j] >> sub_shift) & (nsub - 1)] > kRankMax) *flag = 1u;
+                }
+            }
+        }
+        __syncthreads();
+        if (*flag != 0) {
+            sort_range_merge(dst + lo, src + lo, m, buf, skeys + lo, sidx + lo);
+            continue;
+        }
+        for (uint32_t j = threadIdx.x; j < m; j += kLocalThreads) {
+            const unsigned long long v = buf[j];
+            const uint32_t sb = (uint32_t)(v >> sub_shift) & (nsub - 1);
+            const uint32_t b = start[sb] - lo, c = cnt[sb];
+            // 8 independent LDS reads per step: the loop is bound by LDS
+            // latency, not by the compares
+            uint32_t rank = 0, x = 0;
+            for (; x + 8 <= c; x += 8) {
+                unsigned long long a[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a[u] = buf[b + x + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) rank += a[u] < v ? 1u : 0u;
+            }
+            for (; x < c; ++x) rank += buf[b + x] < v ? 1u : 0u;
+            skeys[start[sb] + rank] = (uint32_t)(v >> 32);
+            sidx[start[sb] + rank] = (uint32_t)v;
+        }
+        __syncthreads();  // buf is reloaded by the next window
+    }
+}
+
 // ---- local: one bucket per block, LDS MSD split + rank inside each sub-bucket
 __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, uint32_t* __restrict__ acc,
                                                          const uint32_t* __restrict__ bases,
@@ -361,37 +526,10 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, uint32_
             return;
         }
     }
-    // crowded or oversized bucket: LDS chunk sorts, then pairwise merges through `alt`
-    constexpr uint32_t kChunk = 2 * kFastCap >= 2048 ? 2048 : 1024;
-    for (uint32_t c0 = 0; c0 < n; c0 += kChunk) {
-        const uint32_t m = min(kChunk, n - c0);
-        for (uint32_t i = threadIdx.x; i < kChunk; i += kLocalThreads) buf[i] = i < m ? src[c0 + i] : ~0ull;
-        __syncthreads();
-        bitonic_lds(buf, kChunk);
-        for (uint32_t i = threadIdx.x; i < m; i += kLocalThreads) src[c0 + i] = buf[i];
-        __syncthreads();
-    }
-    unsigned long long* x = src;
-    unsigned long long* y = alt + b0;
-    for (uint32_t w = kChunk; w < n; w <<= 1) {
-        for (uint32_t e = threadIdx.x; e < n; e += kLocalThreads) {
-            const uint32_t r0 = (e / (2 * w)) * (2 * w);
-            const uint32_t mid = min(r0 + w, n), end = min(r0 + 2 * w, n);
-            const unsigned long long v = x[e];
-            const uint32_t o = e < mid ? (e - r0) + lower_bound_u64(x + mid, end - mid, v)
-                                       : (e - mid) + lower_bound_u64(x + r0, mid - r0, v);
-            y[r0 + o] = v;
-        }
-        __syncthreads();
-        unsigned long long* t = x;
-        x = y;
-        y = t;
-    }
-    for (uint32_t i = threadIdx.x; i < n; i += kLocalThreads) {
-        const unsigned long long v = x[i];
-        skeys[b0 + i] = (uint32_t)(v >> 32);
-        sidx[b0 + i] = (uint32_t)v;
-    }
+    // Oversized bucket, or a crowded sub-bucket: a second MSD split on the
+    // same next key bits, through the scratch `alt`, then windows of
+    // consecutive sub-buckets staged in LDS and placed by rank.
+    large_bucket(src, alt + b0, n, sub_shift, nsub, skeys + b0, sidx + b0, buf, cnt, start, sh, &crowded);
 }
 
 struct Layout {

@@ -128,3 +128,46 @@ def test_empty_scene():
     rays = gpr.gen_rays(1, 1, 100)
     h = s.trace(rays)
     assert np.all(h["id"] == -1)
+
+
+def skewed_keys(seed=12):
+    """Clustered 30-bit keys: most top-digit buckets empty, a few far above the
+    LDS bucket size, crowded sub-buckets, one sub-bucket of 5000 keys (above
+    the rank limit) and runs of equal keys."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for c in range(48):  # 48 clusters of 2^12-wide key ranges
+        base = int(rng.integers(0, 1 << 30 - 12)) << 12
+        parts.append(base + rng.integers(0, 1 << 12, int(rng.integers(500, 9000))))
+    parts.append((7 << 21) + rng.integers(0, 1 << 9, 5000))  # one sub-bucket of 5000
+    parts.append(np.full(2500, 12345678))  # equal keys
+    parts.append(rng.integers(0, 1 << 30, 20000))  # some uniform background
+    k = np.concatenate(parts).astype(np.uint32)
+    return k[rng.permutation(len(k))]
+
+
+def test_sort_skewed_keys_device_entry_point():
+    import torch
+    for seed in (12, 13):
+        keys = skewed_keys(seed)
+        n = len(keys)
+        print(f"[bvh] skewed sort n={n}", flush=True)
+        dk = torch.from_numpy(keys.view(np.int32)).cuda()
+        sk = torch.empty_like(dk)
+        si = torch.empty_like(dk)
+        gpr.sort_pairs_device(dk.data_ptr(), sk.data_ptr(), si.data_ptr(), n)
+        torch.cuda.synchronize()
+        osk, osi = O.sort_pairs(keys)
+        assert np.array_equal(sk.cpu().numpy().view(np.uint32), osk)
+        assert np.array_equal(si.cpu().numpy().view(np.uint32), osi)
+
+
+@pytest.mark.parametrize("name", ["sphere_grid", "mix_1m"])
+def test_build_skewed_scenes(name):
+    # the sphere grid of the full render (253,958 triangles: 88 % of its keys
+    # fall in 32 top-digit buckets far above the LDS bucket size) and a
+    # 1M-triangle sphere grid + soup scene
+    s = gpr.Scene.sphere_grid(64, 36, 32) if name == "sphere_grid" else gpr.Scene.mix(1011, 1 << 20, 0.5)
+    f = s.flat()
+    print(f"[bvh] {name}: {len(f['pos'])} triangles", flush=True)
+    check_build(f["pos"], s)
