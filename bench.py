@@ -42,12 +42,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays in the CPU baseline sample")
     ap.add_argument("--cpu-sample-1core", type=int, default=1 << 16, help="rays in the 1-core CPU baseline sample")
-    ap.add_argument("--only", choices=["all", "trace", "sort", "copy"], default="all",
+    ap.add_argument("--only", choices=["all", "trace", "sort", "copy", "gather"], default="all",
                     help="profiling runs: restrict the GPU work to one kernel family")
-    ap.add_argument("--workload", choices=["raycast", "render", "batch", "serve"], default="raycast",
+    ap.add_argument("--workload", choices=["raycast", "render", "batch", "serve", "build"], default="raycast",
                     help="raycast: 1M-triangle LBVH + ray query (the headline metric, with a served-stream "
                          "segment); render: 1920x1080 sphere-grid path tracer; batch: 256 scenes in one "
-                         "run; serve: the Poisson request stream")
+                         "run; serve: the Poisson request stream; build: LBVH builds of the skewed scenes")
     ap.add_argument("--requests", type=int, default=0,
                     help="cap on the served requests (default: every arrival of the trace)")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
@@ -57,6 +57,9 @@ def parse():
                     help="raycast run without the host-buffer (PCIe-inclusive) timing, so that a kernel "
                          "trace holds only the resident 4M-ray traversal launches")
     ap.add_argument("--no-serve", action="store_true", help="raycast run without the served-stream segment")
+    ap.add_argument("--no-skewed-builds", action="store_true",
+                    help="raycast run without the skewed-scene LBVH builds (so a kernel trace holds only the "
+                         "1M-triangle soup's build and sorts)")
     ap.add_argument("--capacity", type=int, default=1 << 24, help="paths per pass of the served / batched engine")
     ap.add_argument("--trace-cfg", default="64,16,64,1",
                     help="block,stack,chunk,refill of the --only trace profiling run (the tuned configuration)")
@@ -82,6 +85,19 @@ def copy_calibration(torch, nbytes=1 << 30, reps=5):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+
+
+def gather_calibration(torch, rows=1 << 24, n=1 << 23, reps=5):
+    """Random 128-B row reads of known size from a 2 GiB table (past the 256 MB
+    Infinity Cache): the reference point for FETCH_SIZE on a kernel whose
+    reads are scattered whole lines, like the traversal's node fetches."""
+    src = torch.empty(rows, 32, dtype=torch.float32, device="cuda")
+    idx = torch.randint(0, rows, (n,), device="cuda")
+    dst = torch.empty(n, 32, dtype=torch.float32, device="cuda")
+    for _ in range(reps):
+        torch.index_select(src, 0, idx, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"gather_rows": n, "row_bytes": 128, "index_bytes": 8, "reps": reps}), flush=True)
 
 
 def host_cpu_info():
@@ -454,6 +470,36 @@ def serve_latency_under_capacity(a, gpr, np_, torch, rate=30.0, seconds=10.0, n_
             "p99_ms": round(float(np.percentile(lat, 99)), 1)}
 
 
+def skewed_builds(gpr, reps=10):
+    """LBVH build stage times (ms, median of `reps` rebuilds after 2) of the
+    scenes whose Morton keys are clustered: the 1920x1080 render's sphere grid
+    (253,958 triangles) and a 4M-triangle sphere-grid scene of the request
+    stream."""
+    out = {}
+    for name, mk in (("sphere_grid_253958_tris", lambda: gpr.Scene.sphere_grid(1920, 1080, 32)),
+                     ("sphere_grid_4032006_tris", lambda: gpr.Scene.mix(2005, 4000000, 0.0))):
+        s = mk()
+        ts = []
+        for _ in range(reps + 2):
+            s.build()
+            ts.append(s.timing().tolist())
+        m = np.median(np.array(ts[2:]), axis=0)
+        out[name] = {"bounds_morton": round(float(m[0]), 4), "sort": round(float(m[1]), 4),
+                     "tree_refit": round(float(m[2]), 4), "layouts": round(float(m[3]), 4),
+                     "total": round(float(m[4]), 4)}
+        s.free()
+    return out
+
+
+def run_build(a, gpr, np_, torch):
+    res = skewed_builds(gpr, reps=max(1, a.steps))
+    return {"metric": "LBVH build ms (skewed scenes)", "value": res["sphere_grid_253958_tris"]["total"],
+            "unit": "ms", "n_gpus": 1, "steps": max(1, a.steps), "warmup": 2,
+            "ms_per_step": res["sphere_grid_253958_tris"]["total"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scenes)",
+            "config": {"workload": "lbvh_build_sphere_grids"}, "builds": res}
+
+
 def run_serve(a, gpr, np_, torch):
     res = serve_stream(a, gpr, np_, torch, a.serve_seconds, a.scenes or 64, max_requests=a.requests)
     out = {"metric": "served renders/s (Poisson request stream)", "value": res["renders_per_s"],
@@ -481,11 +527,14 @@ def main():
     stream = torch.cuda.current_stream()
     gpr.set_stream(stream.cuda_stream)
     if a.workload != "raycast":
-        fn = {"render": run_render, "batch": run_batch, "serve": run_serve}[a.workload]
+        fn = {"render": run_render, "batch": run_batch, "serve": run_serve, "build": run_build}[a.workload]
         print(json.dumps(fn(a, gpr, np, torch)), flush=True)
         return
     if a.only == "copy":
         copy_calibration(torch)
+        return
+    if a.only == "gather":
+        gather_calibration(torch)
         return
 
     print("[bench] generating inputs", file=sys.stderr, flush=True)
@@ -697,6 +746,9 @@ def main():
     }
     if host is not None:
         out["host_buffers"] = host
+    if not a.no_skewed_builds:
+        print("[bench] skewed-scene builds", file=sys.stderr, flush=True)
+        out["build_ms_skewed_scenes"] = skewed_builds(gpr)
     if not a.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(pos, rays_np, min(a.cpu_sample, N_RAYS),
