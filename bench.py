@@ -66,12 +66,13 @@ def parse():
     return ap.parse_args()
 
 
-def pmc_traffic(key):
+def pmc_traffic(key, field="bytes"):
     """HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
-    passes (copy-calibrated), committed under profiles/; None if absent."""
+    passes (scaled by calibration kernels of known size), committed under
+    profiles/; None if absent."""
     try:
         with open(os.path.join(ROOT, "profiles", "hbm_traffic_pmc.json")) as f:
-            return json.load(f)[key]["bytes"]
+            return json.load(f)[key][field]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -713,13 +714,17 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": pmc_traffic("closest_hit_tuned"),
+                     "traffic_kernel": pmc_traffic("closest_hit_tuned", "kernel"),
+                     "traffic_note": "FETCH_SIZE x 1.063 (a random 128-B line gather of known size) + WRITE_SIZE; "
+                                     "L2-to-fabric bytes, most of them Infinity Cache hits",
                      "kernel": f"k_trace_{'refill' if tuned[0].refill else 'fast'}<false, {tuned[0].block}, {tuned[0].stack}>",
                      "bytes_per_ray": round(bytes_per_ray, 2),
                      "bytes_formula": "32 ray + 16 hit + 128 * BVH4 node fetches + 48 * triangle tests, per ray, "
                                       "counted on the device (gpr_trace_bvh4_stats)",
                      "mean_bvh4_nodes": round(n_node4, 3), "mean_tri_tests": round(n_tri4, 3),
-                     "limiter": "VALU issue (~80 % busy, ~39 % of lanes active) and node-fetch latency "
-                                "(L2 hit ~30 %): most node lines come from the MALL, not HBM",
+                     "limiter": "VALU issue (79 % busy, 39 % of lanes active per instruction) and node-fetch "
+                                "latency (L2 hit 29.6 %); 2.9 TB/s of L2-to-fabric traffic, mostly from the "
+                                "Infinity Cache: not HBM bandwidth",
                      "reference_order_diagnostic": {"bytes_per_ray": round(bytes_per_ray_ref, 2),
                                                     "mean_nodes": round(n_node, 3), "mean_tris": round(n_tri, 3),
                                                     "formula": "32 + 16 + 32 * nodes + 36 * tris of the "
