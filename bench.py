@@ -139,7 +139,7 @@ def median_time(fn, runs=5):
     return float(np.median(ts)), ts
 
 
-def cpu_baseline(pos, rays_np, sample, sample_1core, gpu_ref_hits):
+def cpu_baseline(pos, rays_np, sample, sample_1core, gpu_ref_hits, cpu_nodes):
     """The CPU restatement of the reference path, -O3 -march=native on this
     host: the LBVH build single-threaded as the reference's LBVH_build runs
     (bvh.h:186-221), and the reference-order closest-hit traversal on the
@@ -158,6 +158,7 @@ def cpu_baseline(pos, rays_np, sample, sample_1core, gpu_ref_hits):
     same = bool(np.array_equal(box["hits"].view(np.uint32), gpu_ref_hits[:sample].view(np.uint32)))
     r1 = rays_np[:sample_1core]
     one_s, _ = median_time(lambda: O.trace(nodes, pos, r1, nthreads=1))
+    cpu_nodes.append(nodes)
     return {"value": round(len(rays) / trav_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
             "kind": "port",
             "sample": f"{len(rays)} of the {len(rays_np)} closest-hit rays, reference-order traversal of the "
@@ -651,6 +652,7 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     ms_closest = e0.elapsed_time(e1) / a.steps
+    overflow = gpr.trace_overflow_count()  # rays whose stack overflowed in the last launch (must be 0)
     hits = d_hits.cpu().numpy().view(gpr.HIT_DTYPE)
     hit_frac = float((hits["id"] != -1).mean())
 
@@ -754,6 +756,7 @@ def main():
                             "note": "node / triangle counts of the immediate-leaf walk; the timed kernel "
                                     "postpones leaves into wave-wide rounds"},
         "closest_hit_fraction": round(hit_frac, 4),
+        "stack_overflow_rays": int(overflow),
         "build_ms": {"bounds_morton": round(build_ms[0], 4), "sort": round(build_ms[1], 4),
                      "tree_refit": round(build_ms[2], 4), "layouts": round(build_ms[3], 4),
                      "total": round(build_ms[4], 4)},
@@ -765,9 +768,13 @@ def main():
         out["build_ms_skewed_scenes"] = skewed_builds(gpr)
     if not a.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        cpu_nodes = []
         out["cpu_baseline"] = cpu_baseline(pos, rays_np, min(a.cpu_sample, N_RAYS),
-                                           min(a.cpu_sample_1core, N_RAYS), ref_hits)
+                                           min(a.cpu_sample_1core, N_RAYS), ref_hits, cpu_nodes)
         out["build_ms"]["cpu_reference_path_1thread"] = out["cpu_baseline"]["lbvh_build_ms_1thread"]
+        # the tree the GPU build timed above is the reference's tree, node for node
+        out["build_ms"]["tree_equal_cpu_reference"] = bool(
+            np.array_equal(scene.ref_layout().view(np.uint32), cpu_nodes[0].view(np.uint32)))
     if not a.no_serve:
         print("[bench] served request stream", file=sys.stderr, flush=True)
         del scene, d_rays, d_shadow, d_hits

@@ -314,7 +314,8 @@ int gpr_trace_overflow_count(uint64_t* count);
  * host-buffer traces keep for the calling thread's device and hip_stream
  * (NULL = null stream), after synchronizing that stream; the next call on the
  * stream allocates them again.  For applications that create streams per
- * request or per thread.  No counterpart in the reference (one stream). */
+ * request or per thread; no other call may use the stream meanwhile.  No
+ * counterpart in the reference (one stream). */
 int gpr_release_stream(void* hip_stream);
 
 /* ----------------------------------------------------------- renderer --- */
