@@ -204,9 +204,10 @@ def run_render(a, gpr, np, torch):
            "rays_per_sample": rays_per_sample,
            "mrays_per_s_all_rays": round(rays_per_sample / (ms * 1e-3) / 1e6, 2)}
     if not a.no_cpu_baseline:
+        os.environ["GPR_ORACLE_LIB"] = native_oracle()
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_py as O
-        cores = max(1, min(os.cpu_count() or 1, 16))
+        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
         # a bounded sample: 1 spp of the same scene at 1920x1080
         sw, sh_ = 1920, 1080
         small = gpr.Scene.sphere_grid(sw, sh_, 32)
@@ -220,7 +221,8 @@ def run_render(a, gpr, np, torch):
         out["cpu_baseline"] = {"value": round(sw * sh_ / cdt / 1e6, 4), "unit": "Mpaths/s", "cores": cores,
                                "kind": "port",
                                "sample": f"1 spp of the same scene at {sw}x{sh_}, depth 8, PCG, CPU restatement "
-                                         f"(depth-first per path, {cores} threads), {cdt:.2f} s"}
+                                         f"(-O3 -march=native, depth-first per path, {cores} threads), {cdt:.2f} s",
+                               "host": host_cpu_info()}
     return out
 
 
