@@ -36,6 +36,11 @@ namespace {
 
 constexpr int kMaxDigits = 2048;    // top-digit buckets (11 bits at most)
 constexpr int kQuarters = 4;
+// Reservation groups: tile t reserves its range of each bucket from the
+// counter of group t % kGroups, so each counter takes ntiles / kGroups
+// returning atomics instead of ntiles (device-scope atomics on one address
+// serialise); a bucket holds the groups' ranges one after another.
+constexpr int kGroups = 8;
 constexpr int kTile = 16384;        // keys per histogram tile
 constexpr int kSubTile = kTile / kQuarters;  // keys per scatter tile
 constexpr int kTileThreads = 1024;
@@ -120,7 +125,7 @@ __global__ __launch_bounds__(kTileThreads) void k_hist(const uint32_t* __restric
             q[x] = h[x * nd + d];
             tot += q[x];
         }
-        uint32_t r = tot ? atomicAdd(&acc[d], tot) : 0u;
+        uint32_t r = tot ? atomicAdd(&acc[(blockIdx.x % kGroups) * kMaxDigits + d], tot) : 0u;
 #pragma unroll
         for (int x = 0; x < kQuarters; ++x) {
             __hip_atomic_store(&off[((uint64_t)blockIdx.x * kQuarters + x) * nd + d], r, __ATOMIC_RELAXED,
@@ -164,14 +169,31 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __rest
         k[j] = i < n ? keys[i] : 0u;
     }
     const uint32_t per = nd / kTileThreads;  // nd is a multiple of 1024 or below it
-    uint32_t a0, a1 = 0, o0, o1 = 0;
+    // digit d's total over all groups, and the part of it in groups before
+    // this tile's (the tile's range of bucket d follows those groups' ranges)
+    const uint32_t grp = (sub / kQuarters) % kGroups;
+    auto digit_total = [&](uint32_t d, uint32_t* before) {
+        uint32_t g[kGroups], tot = 0, pre = 0;
+#pragma unroll
+        for (int x = 0; x < kGroups; ++x) g[x] = acc[x * kMaxDigits + d];
+#pragma unroll
+        for (int x = 0; x < kGroups; ++x) {
+            pre += (uint32_t)x < grp ? g[x] : 0u;
+            tot += g[x];
+        }
+        *before = pre;
+        return tot;
+    };
+    uint32_t a0 = 0, a1 = 0, o0 = 0, o1 = 0, p0 = 0, p1 = 0;
     const uint32_t* offs = off + (uint64_t)sub * nd;
     if (per <= 1) {
-        a0 = threadIdx.x < nd ? acc[threadIdx.x] : 0u;
-        o0 = threadIdx.x < nd ? offs[threadIdx.x] : 0u;
+        if (threadIdx.x < nd) {
+            a0 = digit_total(threadIdx.x, &p0);
+            o0 = offs[threadIdx.x];
+        }
     } else {
-        a0 = acc[2 * threadIdx.x];
-        a1 = acc[2 * threadIdx.x + 1];
+        a0 = digit_total(2 * threadIdx.x, &p0);
+        a1 = digit_total(2 * threadIdx.x + 1, &p1);
         o0 = offs[2 * threadIdx.x];
         o1 = offs[2 * threadIdx.x + 1];
     }
@@ -200,14 +222,14 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __rest
     if (per <= 1) {
         if (threadIdx.x < nd) {
             lcnt[threadIdx.x] = lex;
-            gpos[threadIdx.x] = ex + o0 - lex;
+            gpos[threadIdx.x] = ex + p0 + o0 - lex;
             if (sub == 0) bases[threadIdx.x] = ex;
         }
     } else {
         lcnt[2 * threadIdx.x] = lex;
         lcnt[2 * threadIdx.x + 1] = lex + l0;
-        gpos[2 * threadIdx.x] = ex + o0 - lex;
-        gpos[2 * threadIdx.x + 1] = ex + a0 + o1 - (lex + l0);
+        gpos[2 * threadIdx.x] = ex + p0 + o0 - lex;
+        gpos[2 * threadIdx.x + 1] = ex + a0 + p1 + o1 - (lex + l0);
         if (sub == 0) {
             bases[2 * threadIdx.x] = ex;
             bases[2 * threadIdx.x + 1] = ex + a0;
@@ -440,10 +462,12 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, uint32_
     __shared__ uint32_t sh[kLocalThreads / 64];
     __shared__ uint32_t crowded;
     const uint32_t d = blockIdx.x;
-    const uint32_t n = acc[d];
+    uint32_t n = 0;
+#pragma unroll
+    for (int x = 0; x < kGroups; ++x) n += acc[x * kMaxDigits + d];
     const uint32_t b0 = bases[d];
-    __syncthreads();  // every lane has read the bucket's total ...
-    if (threadIdx.x == 0) acc[d] = 0u;  // ... which the next call starts from zero
+    __syncthreads();  // every lane has read the bucket's totals ...
+    if (threadIdx.x < kGroups) acc[threadIdx.x * kMaxDigits + d] = 0u;  // ... which the next call starts from zero
     if (n == 0) return;
     unsigned long long* src = tmp + b0;
     const uint32_t sub_bits = min(10u, shift);
@@ -550,7 +574,7 @@ constexpr Offsets offsets(uint32_t n) {
     const uint64_t nk = n ? n : 1;
     Offsets o{};
     o.acc = 0;
-    o.bases = align256(o.acc + 4ull * kMaxDigits);
+    o.bases = align256(o.acc + 4ull * kMaxDigits * kGroups);
     o.off = align256(o.bases + 4ull * kMaxDigits);
     o.tmp = align256(o.off + 4ull * kMaxDigits * kQuarters * ntiles);
     o.alt = align256(o.tmp + 8ull * nk);
@@ -581,7 +605,7 @@ size_t sort_scratch_bytes(uint32_t n) { return layout(n, nullptr, nullptr); }
 int sort_prepare(void* scratch, size_t bytes, hipStream_t st) {
     // the digit totals (the first array of the layout) start cleared; afterwards
     // every call clears them after use
-    GPR_HIP(hipMemsetAsync(scratch, 0, std::min<size_t>(bytes, 4 * kMaxDigits), st));
+    GPR_HIP(hipMemsetAsync(scratch, 0, std::min<size_t>(bytes, 4 * kMaxDigits * kGroups), st));
     return GPR_OK;
 }
 
@@ -657,7 +681,7 @@ int gpr_sort_pairs_device_noalloc(const uint32_t* keys, uint32_t* sk, uint32_t* 
     GPR_TRY(noalloc_scratch(&p));
     // the digit totals start at zero whatever an earlier call left in the array
     hipLaunchKernelGGL(k_clear_words, dim3(2), dim3(1024), 0, stream(), reinterpret_cast<uint32_t*>(p),
-                       (uint32_t)kMaxDigits);
+                       (uint32_t)(kMaxDigits * kGroups));
     GPR_HIP(hipGetLastError());
     return sort_pairs(keys, sk, si, n, p, kNoAllocBytes, stream());
 }
