@@ -97,6 +97,29 @@ GPR_D uint32_t scan_incl(uint32_t v, uint32_t* sh, uint32_t* total) {
     return v + add;
 }
 
+// inclusive scans of two values per thread over a block of NT threads,
+// sharing their barriers
+template <int NT>
+GPR_D void scan_incl2(uint32_t& a, uint32_t& b, uint32_t* sh /* 2 * NT / 64 */) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    a = wave_scan_incl(a);
+    b = wave_scan_incl(b);
+    if (lane == 63) {
+        sh[2 * wave] = a;
+        sh[2 * wave + 1] = b;
+    }
+    __syncthreads();
+    uint32_t adda = 0, addb = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+        adda += w < wave ? sh[2 * w] : 0u;
+        addb += w < wave ? sh[2 * w + 1] : 0u;
+    }
+    __syncthreads();
+    a += adda;
+    b += addb;
+}
+
 // ---- hist: 16384-key tiles, per-quarter histograms, one reservation per digit
 __global__ __launch_bounds__(kTileThreads) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                        uint32_t shift, uint32_t nd,
@@ -157,7 +180,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __rest
     __shared__ uint32_t gpos[kMaxDigits];  // global slot of digit d's first element, minus its local start
     __shared__ uint32_t lcnt[kMaxDigits];  // digit counts of the quarter, then local starts
     __shared__ unsigned long long stage[kSubTile];
-    __shared__ uint32_t sh[kTileThreads / 64];
+    __shared__ uint32_t sh[2 * kTileThreads / 64];
     const uint32_t sub = quarter_of_block(blockIdx.x);
     const uint64_t b0 = (uint64_t)sub * kSubTile;
     if (b0 >= n) return;  // padding blocks of the XCD mapping
@@ -198,10 +221,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __rest
         o1 = offs[2 * threadIdx.x + 1];
     }
     for (uint32_t d = threadIdx.x; d < nd; d += kTileThreads) lcnt[d] = 0;
-    // bucket bases: exclusive scan of the digit totals (its barriers also
-    // order the lcnt clear before the rank atomics)
-    uint32_t tot;
-    const uint32_t ex = scan_incl<kTileThreads>(a0 + a1, sh, &tot) - (a0 + a1);
+    __syncthreads();
     // rank of every key among the quarter's keys of its digit
     uint32_t r[kSubTile / kTileThreads];
 #pragma unroll
@@ -217,8 +237,12 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __rest
         l0 = lcnt[2 * threadIdx.x];
         l1 = lcnt[2 * threadIdx.x + 1];
     }
-    uint32_t ltot;
-    const uint32_t lex = scan_incl<kTileThreads>(l0 + l1, sh, &ltot) - (l0 + l1);
+    // bucket bases (exclusive scan of the digit totals) and the quarter's
+    // digit starts (exclusive scan of its digit counts) in one pass
+    uint32_t ex = a0 + a1, lex = l0 + l1;
+    scan_incl2<kTileThreads>(ex, lex, sh);
+    ex -= a0 + a1;
+    lex -= l0 + l1;
     if (per <= 1) {
         if (threadIdx.x < nd) {
             lcnt[threadIdx.x] = lex;
