@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __rest
                                                           uint32_t shift, uint32_t nd,
                                                           const uint32_t* __restrict__ acc,
                                                           const uint32_t* __restrict__ off,
-                                                          uint32_t* __restrict__ bases,
+                                                          uint2* __restrict__ bases,
                                                           unsigned long long* __restrict__ tmp) {
     __shared__ uint32_t gpos[kMaxDigits];  // global slot of digit d's first element, minus its local start
     __shared__ uint32_t lcnt[kMaxDigits];  // digit counts of the quarter, then local starts
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __rest
         if (threadIdx.x < nd) {
             lcnt[threadIdx.x] = lex;
             gpos[threadIdx.x] = ex + p0 + o0 - lex;
-            if (sub == 0) bases[threadIdx.x] = ex;
+            if (sub == 0) bases[threadIdx.x] = make_uint2(ex, a0);
         }
     } else {
         lcnt[2 * threadIdx.x] = lex;
@@ -255,8 +255,8 @@ __global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __rest
         gpos[2 * threadIdx.x] = ex + p0 + o0 - lex;
         gpos[2 * threadIdx.x + 1] = ex + a0 + p1 + o1 - (lex + l0);
         if (sub == 0) {
-            bases[2 * threadIdx.x] = ex;
-            bases[2 * threadIdx.x + 1] = ex + a0;
+            bases[2 * threadIdx.x] = make_uint2(ex, a0);
+            bases[2 * threadIdx.x + 1] = make_uint2(ex + a0, a1);
         }
     }
     __syncthreads();
@@ -475,7 +475,7 @@ GPR_D void large_bucket(unsigned long long* src, unsigned long long* dst, uint32
 
 // ---- local: one bucket per block, LDS MSD split + rank inside each sub-bucket
 __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, uint32_t* __restrict__ acc,
-                                                         const uint32_t* __restrict__ bases,
+                                                         const uint2* __restrict__ bases,
                                                          unsigned long long* __restrict__ tmp,
                                                          unsigned long long* __restrict__ alt,
                                                          uint32_t* __restrict__ skeys,
@@ -486,12 +486,11 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, uint32_
     __shared__ uint32_t sh[kLocalThreads / 64];
     __shared__ uint32_t crowded;
     const uint32_t d = blockIdx.x;
-    uint32_t n = 0;
-#pragma unroll
-    for (int x = 0; x < kGroups; ++x) n += acc[x * kMaxDigits + d];
-    const uint32_t b0 = bases[d];
-    __syncthreads();  // every lane has read the bucket's totals ...
-    if (threadIdx.x < kGroups) acc[threadIdx.x * kMaxDigits + d] = 0u;  // ... which the next call starts from zero
+    // the bucket's base and size, as the scatter pass computed them
+    const uint2 bn = bases[d];
+    const uint32_t b0 = bn.x, n = bn.y;
+    // the reservation counters start from zero in the next call
+    if (threadIdx.x < kGroups) acc[threadIdx.x * kMaxDigits + d] = 0u;
     if (n == 0) return;
     unsigned long long* src = tmp + b0;
     const uint32_t sub_bits = min(10u, shift);
@@ -582,7 +581,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, uint32_
 
 struct Layout {
     uint32_t* acc;
-    uint32_t* bases;
+    uint2* bases;     // per bucket: first slot, size
     uint32_t* off;
     unsigned long long* tmp;
     unsigned long long* alt;
@@ -599,7 +598,7 @@ constexpr Offsets offsets(uint32_t n) {
     Offsets o{};
     o.acc = 0;
     o.bases = align256(o.acc + 4ull * kMaxDigits * kGroups);
-    o.off = align256(o.bases + 4ull * kMaxDigits);
+    o.off = align256(o.bases + 8ull * kMaxDigits);
     o.tmp = align256(o.off + 4ull * kMaxDigits * kQuarters * ntiles);
     o.alt = align256(o.tmp + 8ull * nk);
     o.total = o.alt + 8ull * nk + 256;
@@ -609,7 +608,7 @@ size_t layout(uint32_t n, char* base, Layout* L) {
     const Offsets o = offsets(n);
     if (L) {
         L->acc = reinterpret_cast<uint32_t*>(base + o.acc);
-        L->bases = reinterpret_cast<uint32_t*>(base + o.bases);
+        L->bases = reinterpret_cast<uint2*>(base + o.bases);
         L->off = reinterpret_cast<uint32_t*>(base + o.off);
         L->tmp = reinterpret_cast<unsigned long long*>(base + o.tmp);
         L->alt = reinterpret_cast<unsigned long long*>(base + o.alt);
