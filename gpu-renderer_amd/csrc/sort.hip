@@ -126,18 +126,32 @@ __global__ __launch_bounds__(kTileThreads) void k_hist(const uint32_t* __restric
                                                        uint32_t* __restrict__ acc,
                                                        uint32_t* __restrict__ off) {
     __shared__ uint32_t h[kQuarters * kMaxDigits];
-    for (uint32_t i = threadIdx.x; i < kQuarters * nd; i += kTileThreads) h[i] = 0;
-    __syncthreads();
+    // the keys are requested first (16-byte loads where the array allows,
+    // thread t taking keys 4t..4t+3 of each quarter), the LDS histograms
+    // cleared while they are in flight
+    static_assert(kSubTile == 4 * kTileThreads, "one 16-byte load per thread and quarter");
     const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    const bool vec = (reinterpret_cast<uintptr_t>(keys) & 15u) == 0;
     uint32_t k[kTile / kTileThreads];
 #pragma unroll
-    for (int j = 0; j < kTile / kTileThreads; ++j) {
-        const uint64_t i = base + j * kTileThreads + threadIdx.x;
-        k[j] = i < n ? keys[i] : 0xffffffffu;
+    for (int x = 0; x < kQuarters; ++x) {
+        const uint64_t i0 = base + (uint64_t)x * kSubTile + 4u * threadIdx.x;
+        if (vec && i0 + 3 < n) {
+            const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+            k[4 * x] = v.x;
+            k[4 * x + 1] = v.y;
+            k[4 * x + 2] = v.z;
+            k[4 * x + 3] = v.w;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) k[4 * x + c] = i0 + c < n ? keys[i0 + c] : 0xffffffffu;
+        }
     }
+    for (uint32_t i = threadIdx.x; i < kQuarters * nd; i += kTileThreads) h[i] = 0;
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kTile / kTileThreads; ++j) {
-        // element j*1024+t lies in quarter (j*1024+t)/4096 = j/4
+        // k[4x + c] lies in quarter x
         if (k[j] != 0xffffffffu) atomicAdd(&h[(j / 4) * nd + ((k[j] >> shift) & (nd - 1))], 1u);
     }
     __syncthreads();
