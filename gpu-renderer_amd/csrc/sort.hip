@@ -40,7 +40,7 @@ constexpr int kQuarters = 4;
 // counter of group t % kGroups, so each counter takes ntiles / kGroups
 // returning atomics instead of ntiles (device-scope atomics on one address
 // serialise); a bucket holds the groups' ranges one after another.
-constexpr int kGroups = 8;
+constexpr int kGroups = 4;
 constexpr int kTile = 16384;        // keys per histogram tile
 constexpr int kSubTile = kTile / kQuarters;  // keys per scatter tile
 constexpr int kTileThreads = 1024;
