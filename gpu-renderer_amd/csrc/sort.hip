@@ -5,17 +5,17 @@
 // The top digit has between 1 and 11 bits so that a bucket holds ~1K keys
 // (10 bits = 1024 buckets at 1M keys).  Three launches, each reading its
 // input once:
-//   hist     16384-key tiles (1024 threads): LDS histograms of the top digit
-//            per 4096-key quarter; every tile reserves its range inside each
+//   hist     8192-key tiles (1024 threads): LDS histograms of the top digit
+//            per 4096-key half; every tile reserves its range inside each
 //            bucket with one returning atomic per non-empty digit and hands
-//            the quarters consecutive sub-ranges.  The atomics make the tile
+//            the halves consecutive sub-ranges.  The atomics make the tile
 //            order inside a bucket arbitrary, which is harmless because the
 //            last pass orders the full (key, index).
-//   scatter  4096-key quarters: bucket bases from an LDS scan of the digit
-//            totals, LDS-atomic ranks inside the quarter, 8-byte
+//   scatter  4096-key halves: bucket bases from an LDS scan of the digit
+//            totals, LDS-atomic ranks inside the half, 8-byte
 //            (key<<32 | index) composites staged in LDS in digit order and
-//            written with consecutive lanes on consecutive slots.  The four
-//            quarters of a histogram tile run on one XCD.
+//            written with consecutive lanes on consecutive slots.  The two
+//            halves of a histogram tile run on one XCD.
 //   local    one block per bucket staged in LDS: MSD split on the next 10 key
 //            bits into 1024 sub-buckets (~1 element each), then every
 //            composite is placed by its rank among its sub-bucket.  Buckets
@@ -35,14 +35,19 @@ namespace gpr {
 namespace {
 
 constexpr int kMaxDigits = 2048;    // top-digit buckets (11 bits at most)
-constexpr int kQuarters = 4;
+constexpr int kSubs = 2;            // scatter sub-tiles per histogram tile
+constexpr int kScatterThreads = 1024;
 // Reservation groups: tile t reserves its range of each bucket from the
 // counter of group t % kGroups, so each counter takes ntiles / kGroups
 // returning atomics instead of ntiles (device-scope atomics on one address
 // serialise); a bucket holds the groups' ranges one after another.
 constexpr int kGroups = 4;
-constexpr int kTile = 16384;        // keys per histogram tile
-constexpr int kSubTile = kTile / kQuarters;  // keys per scatter tile
+constexpr uint32_t kAccWords = kMaxDigits * kGroups;
+// counter of digit d in group g: one group's counters side by side, so the
+// reservations of one wave touch two cache lines (64 digits)
+GPR_D constexpr uint32_t acc_at(uint32_t d, uint32_t g) { return g * kMaxDigits + d; }
+constexpr int kTile = 8192;         // keys per histogram tile
+constexpr int kSubTile = kTile / kSubs;  // keys per scatter tile
 constexpr int kTileThreads = 1024;
 constexpr int kLocalThreads = 512;
 constexpr int kFastCap = 1536;      // bucket size of the LDS fast path (A and B)
@@ -120,22 +125,23 @@ GPR_D void scan_incl2(uint32_t& a, uint32_t& b, uint32_t* sh /* 2 * NT / 64 */) 
     b += addb;
 }
 
-// ---- hist: 16384-key tiles, per-quarter histograms, one reservation per digit
+// ---- hist: 8192-key tiles, per-half histograms, one reservation per digit
 __global__ __launch_bounds__(kTileThreads) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                        uint32_t shift, uint32_t nd,
                                                        uint32_t* __restrict__ acc,
                                                        uint32_t* __restrict__ off) {
-    __shared__ uint32_t h[kQuarters * kMaxDigits];
+    __shared__ uint32_t h[kSubs * kMaxDigits];
     // the keys are requested first (16-byte loads where the array allows,
-    // thread t taking keys 4t..4t+3 of each quarter), the LDS histograms
+    // thread t taking keys 4t..4t+3 of each 4096-key round), the LDS histograms
     // cleared while they are in flight
-    static_assert(kSubTile == 4 * kTileThreads, "one 16-byte load per thread and quarter");
+    constexpr int kRound = 4 * kTileThreads;  // keys of one round of 16-byte loads
+    static_assert(kRound % kSubTile == 0 || kSubTile % kRound == 0, "load rounds tile the sub-tiles");
     const uint64_t base = (uint64_t)blockIdx.x * kTile;
     const bool vec = (reinterpret_cast<uintptr_t>(keys) & 15u) == 0;
     uint32_t k[kTile / kTileThreads];
 #pragma unroll
-    for (int x = 0; x < kQuarters; ++x) {
-        const uint64_t i0 = base + (uint64_t)x * kSubTile + 4u * threadIdx.x;
+    for (int x = 0; x < kTile / kRound; ++x) {
+        const uint64_t i0 = base + (uint64_t)x * kRound + 4u * threadIdx.x;
         if (vec && i0 + 3 < n) {
             const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
             k[4 * x] = v.x;
@@ -147,25 +153,26 @@ __global__ __launch_bounds__(kTileThreads) void k_hist(const uint32_t* __restric
             for (int c = 0; c < 4; ++c) k[4 * x + c] = i0 + c < n ? keys[i0 + c] : 0xffffffffu;
         }
     }
-    for (uint32_t i = threadIdx.x; i < kQuarters * nd; i += kTileThreads) h[i] = 0;
+    for (uint32_t i = threadIdx.x; i < kSubs * nd; i += kTileThreads) h[i] = 0;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kTile / kTileThreads; ++j) {
-        // k[4x + c] lies in quarter x
-        if (k[j] != 0xffffffffu) atomicAdd(&h[(j / 4) * nd + ((k[j] >> shift) & (nd - 1))], 1u);
+        // k[4x + c] is key x * kRound + 4 * threadIdx.x + c of the tile
+        const uint32_t q = ((uint32_t)(j / 4) * kRound + 4u * threadIdx.x) / kSubTile;
+        if (k[j] != 0xffffffffu) atomicAdd(&h[q * nd + ((k[j] >> shift) & (nd - 1))], 1u);
     }
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < nd; d += kTileThreads) {
-        uint32_t q[kQuarters], tot = 0;
+        uint32_t q[kSubs], tot = 0;
 #pragma unroll
-        for (int x = 0; x < kQuarters; ++x) {
+        for (int x = 0; x < kSubs; ++x) {
             q[x] = h[x * nd + d];
             tot += q[x];
         }
-        uint32_t r = tot ? atomicAdd(&acc[(blockIdx.x % kGroups) * kMaxDigits + d], tot) : 0u;
+        uint32_t r = tot ? atomicAdd(&acc[acc_at(d, blockIdx.x % kGroups)], tot) : 0u;
 #pragma unroll
-        for (int x = 0; x < kQuarters; ++x) {
-            __hip_atomic_store(&off[((uint64_t)blockIdx.x * kQuarters + x) * nd + d], r, __ATOMIC_RELAXED,
+        for (int x = 0; x < kSubs; ++x) {
+            __hip_atomic_store(&off[((uint64_t)blockIdx.x * kSubs + x) * nd + d], r, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
             r += q[x];
         }
@@ -175,121 +182,116 @@ __global__ __launch_bounds__(kTileThreads) void k_hist(const uint32_t* __restric
 // ---- scatter: 4096-key sub-tiles, composites into their bucket.
 // Workgroups are dispatched round-robin over the 8 XCDs (block b on XCD b%8)
 // and the L2 of one XCD is not coherent with the others', so a cache line
-// written from two XCDs reaches HBM as two partial writes.  The four quarters
+// written from two XCDs reaches HBM as two partial writes.  The two halves
 // of one histogram tile own consecutive sub-ranges of every bucket; mapping
 // them onto one XCD lets its L2 merge their writes into whole lines.
 constexpr int kXcds = 8;
-GPR_D uint32_t quarter_of_block(uint32_t b) {
+GPR_D uint32_t sub_of_block(uint32_t b) {
     const uint32_t g = b / kXcds, x = b % kXcds;
-    const uint32_t t = (g / kQuarters) * kXcds + x;
-    return t * kQuarters + g % kQuarters;
+    const uint32_t t = (g / kSubs) * kXcds + x;
+    return t * kSubs + g % kSubs;
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_scatter(const uint32_t* __restrict__ keys, uint32_t n,
-                                                          uint32_t shift, uint32_t nd,
-                                                          const uint32_t* __restrict__ acc,
-                                                          const uint32_t* __restrict__ off,
-                                                          uint2* __restrict__ bases,
-                                                          unsigned long long* __restrict__ tmp) {
+__global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __restrict__ keys, uint32_t n,
+                                                             uint32_t shift, uint32_t nd,
+                                                             const uint32_t* __restrict__ acc,
+                                                             const uint32_t* __restrict__ off,
+                                                             uint2* __restrict__ bases,
+                                                             unsigned long long* __restrict__ tmp) {
+    constexpr int NT = kScatterThreads, KPT = kSubTile / NT;
+    constexpr int kPer = kMaxDigits / NT;  // digits per thread at most
+    static_assert(kSubTile % NT == 0 && kMaxDigits % NT == 0, "whole keys and digits per thread");
     __shared__ uint32_t gpos[kMaxDigits];  // global slot of digit d's first element, minus its local start
-    __shared__ uint32_t lcnt[kMaxDigits];  // digit counts of the quarter, then local starts
+    __shared__ uint32_t lcnt[kMaxDigits];  // digit counts of the sub-tile, then local starts
     __shared__ unsigned long long stage[kSubTile];
-    __shared__ uint32_t sh[2 * kTileThreads / 64];
-    const uint32_t sub = quarter_of_block(blockIdx.x);
+    __shared__ uint32_t sh[2 * NT / 64];
+    const uint32_t sub = sub_of_block(blockIdx.x);
     const uint64_t b0 = (uint64_t)sub * kSubTile;
     if (b0 >= n) return;  // padding blocks of the XCD mapping
     // every global load is issued before the first dependent use
-    uint32_t k[kSubTile / kTileThreads];
+    uint32_t k[KPT];
 #pragma unroll
-    for (int j = 0; j < kSubTile / kTileThreads; ++j) {
-        const uint64_t i = b0 + j * kTileThreads + threadIdx.x;
+    for (int j = 0; j < KPT; ++j) {
+        const uint64_t i = b0 + j * NT + threadIdx.x;
         k[j] = i < n ? keys[i] : 0u;
     }
-    const uint32_t per = nd / kTileThreads;  // nd is a multiple of 1024 or below it
+    // thread t owns digits per*t .. per*t + per - 1 (nd is a power of two:
+    // either a multiple of NT or below it, then one digit for threads < nd)
+    const uint32_t per = nd >= (uint32_t)NT ? nd / NT : 1u;
+    const bool owns = threadIdx.x * per < nd;
+    const uint32_t d0 = threadIdx.x * per;
     // digit d's total over all groups, and the part of it in groups before
     // this tile's (the tile's range of bucket d follows those groups' ranges)
-    const uint32_t grp = (sub / kQuarters) % kGroups;
-    auto digit_total = [&](uint32_t d, uint32_t* before) {
-        uint32_t g[kGroups], tot = 0, pre = 0;
-#pragma unroll
-        for (int x = 0; x < kGroups; ++x) g[x] = acc[x * kMaxDigits + d];
-#pragma unroll
-        for (int x = 0; x < kGroups; ++x) {
-            pre += (uint32_t)x < grp ? g[x] : 0u;
-            tot += g[x];
-        }
-        *before = pre;
-        return tot;
-    };
-    uint32_t a0 = 0, a1 = 0, o0 = 0, o1 = 0, p0 = 0, p1 = 0;
+    const uint32_t grp = (sub / kSubs) % kGroups;
     const uint32_t* offs = off + (uint64_t)sub * nd;
-    if (per <= 1) {
-        if (threadIdx.x < nd) {
-            a0 = digit_total(threadIdx.x, &p0);
-            o0 = offs[threadIdx.x];
-        }
-    } else {
-        a0 = digit_total(2 * threadIdx.x, &p0);
-        a1 = digit_total(2 * threadIdx.x + 1, &p1);
-        o0 = offs[2 * threadIdx.x];
-        o1 = offs[2 * threadIdx.x + 1];
-    }
-    for (uint32_t d = threadIdx.x; d < nd; d += kTileThreads) lcnt[d] = 0;
-    __syncthreads();
-    // rank of every key among the quarter's keys of its digit
-    uint32_t r[kSubTile / kTileThreads];
+    uint32_t a[kPer], p[kPer], o[kPer];
 #pragma unroll
-    for (int j = 0; j < kSubTile / kTileThreads; ++j) {
-        const uint64_t i = b0 + j * kTileThreads + threadIdx.x;
+    for (int x = 0; x < kPer; ++x) {
+        a[x] = p[x] = o[x] = 0u;
+        if (owns && (uint32_t)x < per) {
+            uint32_t g[kGroups];
+#pragma unroll
+            for (int y = 0; y < kGroups; ++y) g[y] = acc[acc_at(d0 + x, y)];
+#pragma unroll
+            for (int y = 0; y < kGroups; ++y) {
+                p[x] += (uint32_t)y < grp ? g[y] : 0u;
+                a[x] += g[y];
+            }
+            o[x] = offs[d0 + x];
+        }
+    }
+    for (uint32_t d = threadIdx.x; d < nd; d += NT) lcnt[d] = 0;
+    __syncthreads();
+    // rank of every key among the sub-tile's keys of its digit
+    uint32_t r[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+        const uint64_t i = b0 + j * NT + threadIdx.x;
         r[j] = i < n ? atomicAdd(&lcnt[(k[j] >> shift) & (nd - 1)], 1u) : 0u;
     }
     __syncthreads();
-    uint32_t l0, l1 = 0;
-    if (per <= 1) {
-        l0 = threadIdx.x < nd ? lcnt[threadIdx.x] : 0u;
-    } else {
-        l0 = lcnt[2 * threadIdx.x];
-        l1 = lcnt[2 * threadIdx.x + 1];
+    uint32_t l[kPer], ex = 0, lex = 0;
+#pragma unroll
+    for (int x = 0; x < kPer; ++x) {
+        l[x] = owns && (uint32_t)x < per ? lcnt[d0 + x] : 0u;
+        ex += a[x];
+        lex += l[x];
     }
-    // bucket bases (exclusive scan of the digit totals) and the quarter's
+    // bucket bases (exclusive scan of the digit totals) and the sub-tile's
     // digit starts (exclusive scan of its digit counts) in one pass
-    uint32_t ex = a0 + a1, lex = l0 + l1;
-    scan_incl2<kTileThreads>(ex, lex, sh);
-    ex -= a0 + a1;
-    lex -= l0 + l1;
-    if (per <= 1) {
-        if (threadIdx.x < nd) {
-            lcnt[threadIdx.x] = lex;
-            gpos[threadIdx.x] = ex + p0 + o0 - lex;
-            if (sub == 0) bases[threadIdx.x] = make_uint2(ex, a0);
-        }
-    } else {
-        lcnt[2 * threadIdx.x] = lex;
-        lcnt[2 * threadIdx.x + 1] = lex + l0;
-        gpos[2 * threadIdx.x] = ex + p0 + o0 - lex;
-        gpos[2 * threadIdx.x + 1] = ex + a0 + p1 + o1 - (lex + l0);
-        if (sub == 0) {
-            bases[2 * threadIdx.x] = make_uint2(ex, a0);
-            bases[2 * threadIdx.x + 1] = make_uint2(ex + a0, a1);
+    const uint32_t ta = ex, tl = lex;
+    scan_incl2<NT>(ex, lex, sh);
+    ex -= ta;
+    lex -= tl;
+    if (owns) {
+#pragma unroll
+        for (int x = 0; x < kPer; ++x) {
+            if ((uint32_t)x < per) {
+                lcnt[d0 + x] = lex;
+                gpos[d0 + x] = ex + p[x] + o[x] - lex;
+                if (sub == 0) bases[d0 + x] = make_uint2(ex, a[x]);
+                ex += a[x];
+                lex += l[x];
+            }
         }
     }
     __syncthreads();
     // stage the composites in digit order, then write them with consecutive
     // lanes on consecutive slots of each digit's run
 #pragma unroll
-    for (int j = 0; j < kSubTile / kTileThreads; ++j) {
-        const uint64_t i = b0 + j * kTileThreads + threadIdx.x;
+    for (int j = 0; j < KPT; ++j) {
+        const uint64_t i = b0 + j * NT + threadIdx.x;
         if (i < n)
             stage[lcnt[(k[j] >> shift) & (nd - 1)] + r[j]] = ((unsigned long long)k[j] << 32) | (unsigned long long)i;
     }
     __syncthreads();
     const uint32_t m = (uint32_t)min<uint64_t>(kSubTile, n - b0);
 #pragma unroll
-    for (int j = 0; j < kSubTile / kTileThreads; ++j) {
-        const uint32_t p = j * kTileThreads + threadIdx.x;
-        if (p < m) {
-            const unsigned long long w = stage[p];
-            __hip_atomic_store(&tmp[gpos[(uint32_t)(w >> (32 + shift)) & (nd - 1)] + p], w, __ATOMIC_RELAXED,
+    for (int j = 0; j < KPT; ++j) {
+        const uint32_t q = j * NT + threadIdx.x;
+        if (q < m) {
+            const unsigned long long w = stage[q];
+            __hip_atomic_store(&tmp[gpos[(uint32_t)(w >> (32 + shift)) & (nd - 1)] + q], w, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(kLocalThreads) void k_local(uint32_t shift, uint32_
     const uint2 bn = bases[d];
     const uint32_t b0 = bn.x, n = bn.y;
     // the reservation counters start from zero in the next call
-    if (threadIdx.x < kGroups) acc[threadIdx.x * kMaxDigits + d] = 0u;
+    if (threadIdx.x < kGroups) acc[acc_at(d, threadIdx.x)] = 0u;
     if (n == 0) return;
     unsigned long long* src = tmp + b0;
     const uint32_t sub_bits = min(10u, shift);
@@ -611,9 +613,9 @@ constexpr Offsets offsets(uint32_t n) {
     const uint64_t nk = n ? n : 1;
     Offsets o{};
     o.acc = 0;
-    o.bases = align256(o.acc + 4ull * kMaxDigits * kGroups);
+    o.bases = align256(o.acc + 4ull * kAccWords);
     o.off = align256(o.bases + 8ull * kMaxDigits);
-    o.tmp = align256(o.off + 4ull * kMaxDigits * kQuarters * ntiles);
+    o.tmp = align256(o.off + 4ull * kMaxDigits * kSubs * ntiles);
     o.alt = align256(o.tmp + 8ull * nk);
     o.total = o.alt + 8ull * nk + 256;
     return o;
@@ -642,7 +644,7 @@ size_t sort_scratch_bytes(uint32_t n) { return layout(n, nullptr, nullptr); }
 int sort_prepare(void* scratch, size_t bytes, hipStream_t st) {
     // the digit totals (the first array of the layout) start cleared; afterwards
     // every call clears them after use
-    GPR_HIP(hipMemsetAsync(scratch, 0, std::min<size_t>(bytes, 4 * kMaxDigits * kGroups), st));
+    GPR_HIP(hipMemsetAsync(scratch, 0, std::min<size_t>(bytes, 4ull * kAccWords), st));
     return GPR_OK;
 }
 
@@ -657,8 +659,8 @@ int sort_pairs(const uint32_t* keys, uint32_t* skeys, uint32_t* sidx, uint32_t n
     const uint32_t ntiles = div_up(n, kTile);
     hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kTileThreads), 0, st, keys, n, shift, nd, L.acc, L.off);
     // grid padded to whole groups of kXcds tiles for the XCD mapping
-    const uint32_t nsub_blocks = div_up(ntiles, kXcds) * kXcds * kQuarters;
-    hipLaunchKernelGGL(k_scatter, dim3(nsub_blocks), dim3(kTileThreads), 0, st, keys, n, shift, nd,
+    const uint32_t nsub_blocks = div_up(ntiles, kXcds) * kXcds * kSubs;
+    hipLaunchKernelGGL(k_scatter, dim3(nsub_blocks), dim3(kScatterThreads), 0, st, keys, n, shift, nd,
                        L.acc, L.off, L.bases, L.tmp);
     hipLaunchKernelGGL(k_local, dim3(nd), dim3(kLocalThreads), 0, st, shift, L.acc, L.bases, L.tmp,
                        L.alt, skeys, sidx);
@@ -718,7 +720,7 @@ int gpr_sort_pairs_device_noalloc(const uint32_t* keys, uint32_t* sk, uint32_t* 
     GPR_TRY(noalloc_scratch(&p));
     // the digit totals start at zero whatever an earlier call left in the array
     hipLaunchKernelGGL(k_clear_words, dim3(2), dim3(1024), 0, stream(), reinterpret_cast<uint32_t*>(p),
-                       (uint32_t)(kMaxDigits * kGroups));
+                       (uint32_t)kAccWords);
     GPR_HIP(hipGetLastError());
     return sort_pairs(keys, sk, si, n, p, kNoAllocBytes, stream());
 }
