@@ -64,8 +64,9 @@ def test_build_1m_soup():
 def test_sort_device_entry_point():
     import torch
     rng = np.random.default_rng(4)
+    # sizes around the 4096-key scatter blocks and 8192-key histogram tiles;
     # 4M keys: 2048 top-digit buckets of ~2K keys, above the bucket pass's LDS fast path
-    for n in (1, 5, 4096, 4097, 300001, 1 << 22):
+    for n in (1, 5, 4096, 4097, 8191, 8192, 8193, 12289, 300001, 1 << 22):
         keys = rng.integers(0, 1 << 30, n).astype(np.uint32)
         keys[: n // 3] = keys[0]  # a long run of equal keys
         dk = torch.from_numpy(keys.view(np.int32)).cuda()
@@ -74,8 +75,20 @@ def test_sort_device_entry_point():
         gpr.sort_pairs_device(dk.data_ptr(), sk.data_ptr(), si.data_ptr(), n)
         torch.cuda.synchronize()
         osk, osi = O.sort_pairs(keys)
-        assert np.array_equal(sk.cpu().numpy().view(np.uint32), osk)
-        assert np.array_equal(si.cpu().numpy().view(np.uint32), osi)
+        assert np.array_equal(sk.cpu().numpy().view(np.uint32), osk), n
+        assert np.array_equal(si.cpu().numpy().view(np.uint32), osi), n
+    # keys that do not start on a 16-byte boundary (the histogram pass then
+    # reads them one by one instead of as 16-byte vectors)
+    keys = rng.integers(0, 1 << 30, 300002).astype(np.uint32)
+    dk = torch.from_numpy(keys.view(np.int32)).cuda()
+    sk = torch.empty(300001, dtype=torch.int32, device="cuda")
+    si = torch.empty_like(sk)
+    assert (dk.data_ptr() + 4) % 16 != 0
+    gpr.sort_pairs_device(dk.data_ptr() + 4, sk.data_ptr(), si.data_ptr(), 300001)
+    torch.cuda.synchronize()
+    osk, osi = O.sort_pairs(keys[1:])
+    assert np.array_equal(sk.cpu().numpy().view(np.uint32), osk)
+    assert np.array_equal(si.cpu().numpy().view(np.uint32), osi)
 
 
 def test_sort_graph_replays_interleaved_with_eager_sorts():
