@@ -265,6 +265,7 @@ gpr::SceneView gpr_scene::view() const {
     v.ref = bvh.ref;
     v.bvh4 = bvh.bvh4;
     v.tris = bvh.tris;
+    v.shade = d_shade.as<TriShade>();
     v.root4 = bvh.root4;
     v.nbvh4 = bvh.nbvh4;
     v.nref = bvh.n ? 2 * bvh.n - 1 : 0;
@@ -523,6 +524,7 @@ int gpr_scene_commit(gpr_scene* s) {
         GPR_TRY(upload(s->d_tex, s->textures));
         GPR_TRY(upload(s->d_texels, s->texels));
     }
+    GPR_TRY(pack_shade(s));
     GPR_TRY(s->d_stats.alloc(256));
     if (!s->cam_set) camera_raster_to_camera(s->width, s->height, s->fov, s->r2c);
     GPR_TRY(build_bvh(s));

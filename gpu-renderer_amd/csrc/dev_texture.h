@@ -98,13 +98,16 @@ GPR_D bool has_textures(const gpr_material& m) {
 // Replace the textured fields of m by their lookups at the hit (the BxDF
 // factories, bsdfs/bxdf.h:160-218).  uv: Triangle::setupDifferential
 // (triangle.h:126-134) -- interpolated corner uvs, else the barycentrics.
-GPR_D void resolve_textures(const SceneView& sv, gpr_material& m, uint32_t t, float bu, float bv) {
+GPR_D void resolve_textures(const SceneView& sv, gpr_material& m, uint32_t t, uint32_t flags, float bu,
+                            float bv) {
     float u = bu, v = bv;
-    if (sv.flags[t] & 2u) {
-        const float* q = sv.uv + 6ull * t;
+    if (flags & 2u) {
+        // the corner uvs of the triangle's shading record
+        const float4* r = reinterpret_cast<const float4*>(sv.shade + t);
+        const float4 q = r[5], q2 = r[6];
         const float w0 = 1 - bu - bv;
-        u = q[0] * w0 + q[2] * bu + q[4] * bv;
-        v = q[1] * w0 + q[3] * bu + q[5] * bv;
+        u = q.x * w0 + q.z * bu + q2.x * bv;
+        v = q.y * w0 + q.w * bu + q2.y * bv;
     }
 #pragma unroll
     for (int k = 0; k < GPR_TEX_SLOTS; ++k) {

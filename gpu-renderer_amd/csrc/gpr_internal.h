@@ -96,6 +96,24 @@ struct alignas(16) TriRec {
     float4 c;  // p2.z id(bits) 0 0
 };
 
+// Shading record of one triangle in triangle-id order, one 128-B line: the
+// corner positions and normals, the flags (1: corner normals, 2: corner uvs),
+// the material index and the corner uvs.  Shading reads a hit or light
+// triangle with a few 16-byte loads from one line instead of ~20 scattered
+// 4-byte loads from four arrays; the values are copies, so every result
+// computed from them is unchanged.
+struct alignas(16) TriShade {
+    float4 a;  // p0.x p0.y p0.z p1.x
+    float4 b;  // p1.y p1.z p2.x p2.y
+    float4 c;  // p2.z n0.x n0.y n0.z
+    float4 d;  // n1.x n1.y n1.z n2.x
+    float4 e;  // n2.y n2.z flags(bits) material(bits)
+    float4 f;  // uv0.u uv0.v uv1.u uv1.v
+    float4 g;  // uv2.u uv2.v 0 0
+    float4 h;  // 0
+};
+static_assert(sizeof(TriShade) == 128, "shading record must be one 128-B line");
+
 // Device-side view of a committed scene, passed by value to kernels.
 struct SceneView {
     uint32_t ntri;
@@ -109,6 +127,7 @@ struct SceneView {
     const gpr_ref_node* ref; // 2n-1, reference DFS layout
     const Bvh4Node* bvh4;
     const TriRec* tris;      // sorted-leaf order
+    const TriShade* shade;   // ntri, triangle-id order
     int32_t root4;           // BVH4 root reference (node index or ~leaf)
     uint32_t nbvh4;          // BVH4 node count (bounds every node fetch)
     uint32_t nref;           // reference-layout node count (2n-1)
@@ -184,7 +203,7 @@ struct gpr_scene {
     std::vector<uint32_t> mat, lights;
     std::vector<gpr_material> mats;
     // device copies
-    gpr::DevBuf d_pos, d_nrm, d_flags, d_mat, d_mats, d_lights, d_stats, d_uv, d_tex, d_texels;
+    gpr::DevBuf d_pos, d_nrm, d_flags, d_mat, d_mats, d_lights, d_stats, d_uv, d_tex, d_texels, d_shade;
     gpr::BvhDev bvh;
     bool flattened = false;
     bool committed = false;
@@ -197,6 +216,8 @@ struct gpr_scene {
 namespace gpr {
 // bvh_build.hip
 int build_bvh(gpr_scene* s);
+// render.hip: packs the scene's TriShade records (d_shade) from its uploaded arrays
+int pack_shade(gpr_scene* s);
 // sort.hip
 int sort_pairs(const uint32_t* keys, uint32_t* skeys, uint32_t* sidx, uint32_t n, void* scratch,
                size_t scratch_bytes, hipStream_t st);

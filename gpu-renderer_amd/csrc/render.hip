@@ -228,30 +228,40 @@ GPR_D gpr_hit trace_one(const RenderArgs& a, const SceneView& sv, const Ray3& r,
     return h;
 }
 
-// Light pdf (Triangle::pdf + pdf_lights, triangle.h:156-175, pathtracer_kernel.h:338-348)
-GPR_D float tri_area(const SceneView& sv, uint32_t t) {
-    const float* p = sv.pos + 9ull * t;
-    const v3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
-    return len3(cross3(p1 - p0, p2 - p0)) * 0.5f;
+// One triangle's shading record, read as five 16-byte loads from its line
+// (the corner uvs, read only for textured hits, are loaded by uv_at).
+struct Shade {
+    v3 p0, p1, p2, n0, n1, n2;
+    uint32_t flags, mat;
+};
+GPR_D Shade load_shade(const SceneView& sv, uint32_t t) {
+    const float4* r = reinterpret_cast<const float4*>(sv.shade + t);
+    const float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
+    Shade s;
+    s.p0 = V(a.x, a.y, a.z);
+    s.p1 = V(a.w, b.x, b.y);
+    s.p2 = V(b.z, b.w, c.x);
+    s.n0 = V(c.y, c.z, c.w);
+    s.n1 = V(d.x, d.y, d.z);
+    s.n2 = V(d.w, e.x, e.y);
+    s.flags = __float_as_uint(e.z);
+    s.mat = __float_as_uint(e.w);
+    return s;
 }
+// Light pdf (Triangle::pdf + pdf_lights, triangle.h:156-175, pathtracer_kernel.h:338-348)
+GPR_D float tri_area(const Shade& s) { return len3(cross3(s.p1 - s.p0, s.p2 - s.p0)) * 0.5f; }
 struct Geom {
     v3 p, ns;
 };
 // Triangle::setupDifferential (triangle.h:104-138)
-GPR_D Geom setup_geom(const SceneView& sv, uint32_t t, float u, float v) {
-    const float* p = sv.pos + 9ull * t;
-    const v3 p0 = load3(p), p1 = load3(p + 3), p2 = load3(p + 6);
+GPR_D Geom setup_geom(const Shade& s, float u, float v) {
     Geom g;
-    g.p = p0 * (1 - u - v) + p1 * u + p2 * v;
-    if (sv.flags[t] & 1u) {
-        const float* n = sv.nrm + 9ull * t;
-        g.ns = norm3(load3(n) * (1 - u - v) + load3(n + 3) * u + load3(n + 6) * v);
-    } else {
-        g.ns = norm3(cross3(p0 - p2, p1 - p2));
-    }
+    g.p = s.p0 * (1 - u - v) + s.p1 * u + s.p2 * v;
+    if (s.flags & 1u) g.ns = norm3(s.n0 * (1 - u - v) + s.n1 * u + s.n2 * v);
+    else g.ns = norm3(cross3(s.p0 - s.p2, s.p1 - s.p2));
     return g;
 }
-GPR_D c3 emission_of(const SceneView& sv, uint32_t t) { return loadc(sv.mats[sv.mat[t]].emission); }
+GPR_D c3 emission_of(const SceneView& sv, const Shade& s) { return loadc(sv.mats[s.mat].emission); }
 GPR_D float power_heur(float a, float b) {
     a *= a;
     b *= b;
@@ -346,11 +356,11 @@ GPR_D void extend_finish(const RenderArgs& a, const QueueView& q, uint32_t i, co
         float u_rr = 0.f;
         if (!MULTI && a.rng_mode == GPR_RNG_REFERENCE) u_rr = lcg_next(q.seed[i]);
         if (h.id != -1 && !spec) {
-            const uint32_t t = (uint32_t)h.id;
-            const c3 le = emission_of(sv, t);
+            const Shade ts = load_shade(sv, (uint32_t)h.id);
+            const c3 le = emission_of(sv, ts);
             if (!black(le)) {
-                const Geom lg = setup_geom(sv, t, h.u, h.v);
-                float lpdf = 1.f / tri_area(sv, t);
+                const Geom lg = setup_geom(ts, h.u, h.v);
+                float lpdf = 1.f / tri_area(ts);
                 lpdf /= sv.nlights;
                 const float d = h.t;
                 lpdf *= d * d / fabsf(dot3(r.d, lg.ns));
@@ -534,10 +544,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
         }
         if (hid != -1) {
             const uint32_t t = (uint32_t)hid;
-            const Geom g = setup_geom(sv, t, hh.z, hh.w);
+            const Shade ts = load_shade(sv, t);
+            const Geom g = setup_geom(ts, hh.z, hh.w);
             const v3 wo = -norm3(rd);
-            gpr_material m = sv.mats[sv.mat[t]];
-            if (sv.ntex != 0 && has_textures(m)) resolve_textures(sv, m, t, hh.z, hh.w);
+            gpr_material m = sv.mats[ts.mat];
+            if (sv.ntex != 0 && has_textures(m)) resolve_textures(sv, m, t, ts.flags, hh.z, hh.w);
             const Frame fr = make_frame(g.ns);
             if (bounce == 0 || spec) {
                 const c3 le = loadc(m.emission);
@@ -553,8 +564,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
                 if (lid > ln - 1) lid = ln - 1;
                 const uint32_t lt = sv.lights[lid];
                 const float sa = sqrtf(smp[1]);
-                const Geom lg = setup_geom(sv, lt, 1 - sa, smp[2] * sa);
-                float lpdf = 1.f / tri_area(sv, lt);
+                const Shade ls = load_shade(sv, lt);
+                const Geom lg = setup_geom(ls, 1 - sa, smp[2] * sa);
+                float lpdf = 1.f / tri_area(ls);
                 lpdf /= ln;
                 v3 wi = lg.p - g.p;
                 const float d2 = dot3(wi, wi);
@@ -562,7 +574,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
                 wi = wi / dd;
                 lpdf *= d2 / fabsf(dot3(wi, lg.ns));
                 if (isinf(lpdf)) lpdf = 0.f;
-                const c3 fl = dot3(-wi, lg.ns) > 0 ? emission_of(sv, lt) : C(0.f);
+                const c3 fl = dot3(-wi, lg.ns) > 0 ? emission_of(sv, ls) : C(0.f);
                 c3 fb;
                 float bpdf;
                 bsdf_eval_local(m, to_local(fr, wo), to_local(fr, wi), fb, bpdf);
@@ -962,6 +974,47 @@ int engine_errors(Engine* e, hipStream_t st, uint32_t* flags) {
                            hipMemcpyDeviceToHost, st));
     GPR_HIP(hipStreamSynchronize(st));
     *flags = err;
+    return GPR_OK;
+}
+
+namespace {
+__global__ __launch_bounds__(256) void k_pack_shade(const float* __restrict__ pos, const float* __restrict__ nrm,
+                                                    const uint8_t* __restrict__ flags,
+                                                    const uint32_t* __restrict__ mat,
+                                                    const float* __restrict__ uv, uint32_t n,
+                                                    TriShade* __restrict__ out) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const float* p = pos + 9ull * t;
+    const float* q = nrm + 9ull * t;
+    const uint32_t f = flags[t];
+    float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (uv) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[k] = uv[6ull * t + k];
+    }
+    TriShade r;
+    r.a = make_float4(p[0], p[1], p[2], p[3]);
+    r.b = make_float4(p[4], p[5], p[6], p[7]);
+    r.c = make_float4(p[8], q[0], q[1], q[2]);
+    r.d = make_float4(q[3], q[4], q[5], q[6]);
+    r.e = make_float4(q[7], q[8], __uint_as_float(f), __uint_as_float(mat[t]));
+    r.f = make_float4(w[0], w[1], w[2], w[3]);
+    r.g = make_float4(w[4], w[5], 0.f, 0.f);
+    r.h = make_float4(0.f, 0.f, 0.f, 0.f);
+    out[t] = r;
+}
+}  // namespace
+
+int pack_shade(gpr_scene* s) {
+    const uint32_t n = (uint32_t)s->mat.size();
+    GPR_TRY(s->d_shade.alloc(std::max<size_t>(sizeof(TriShade) * n, 256)));
+    if (n == 0) return GPR_OK;
+    const float* uv = s->uv.empty() || s->textures.empty() ? nullptr : s->d_uv.as<float>();
+    hipLaunchKernelGGL(k_pack_shade, dim3((n + 255) / 256), dim3(256), 0, stream(), s->d_pos.as<float>(),
+                       s->d_nrm.as<float>(), s->d_flags.as<uint8_t>(), s->d_mat.as<uint32_t>(), uv, n,
+                       s->d_shade.as<TriShade>());
+    GPR_HIP(hipGetLastError());
     return GPR_OK;
 }
 

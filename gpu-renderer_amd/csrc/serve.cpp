@@ -48,7 +48,8 @@ size_t scene_bytes(const gpr_scene* s) {
     const size_t n = s->mat.size();
     return align256(36 * n) * 2 + align256(n) + align256(4 * n) + align256(sizeof(gpr_material) * s->mats.size()) +
            align256(4 * std::max<size_t>(1, s->lights.size())) + align256(sizeof(Bvh4Node) * s->bvh.nbvh4) +
-           align256(sizeof(TriRec) * n) + align256(sizeof(gpr_ref_node) * (n ? 2 * n - 1 : 0)) + texture_bytes(s);
+           align256(sizeof(TriRec) * n) + align256(sizeof(gpr_ref_node) * (n ? 2 * n - 1 : 0)) +
+           align256(sizeof(TriShade) * n) + texture_bytes(s);
 }
 
 int arena_add(Arena& A, const gpr_scene* s, hipStream_t st, uint32_t* id) {
@@ -79,7 +80,9 @@ int arena_add(Arena& A, const gpr_scene* s, hipStream_t st, uint32_t* id) {
     a.bvh4 = reinterpret_cast<const Bvh4Node*>(put(v.bvh4, sizeof(Bvh4Node) * s->bvh.nbvh4));
     a.tris = reinterpret_cast<const TriRec*>(put(v.tris, sizeof(TriRec) * n));
     a.ref = reinterpret_cast<const gpr_ref_node*>(put(v.ref, sizeof(gpr_ref_node) * (2 * n - 1)));
-    if (!a.pos || !a.nrm || !a.flags || !a.mat || !a.mats || !a.lights || !a.bvh4 || !a.tris || !a.ref)
+    a.shade = reinterpret_cast<const TriShade*>(put(v.shade, sizeof(TriShade) * n));
+    if (!a.pos || !a.nrm || !a.flags || !a.mat || !a.mats || !a.lights || !a.bvh4 || !a.tris || !a.ref ||
+        !a.shade)
         return fail(GPR_ERR_HIP, "add_scene: arena copy failed");
     if (!s->textures.empty()) {
         a.uv = reinterpret_cast<const float*>(put(v.uv, 24 * n));
