@@ -50,7 +50,7 @@ struct QueueView {
     uint32_t* pix;    // frame-buffer pixel | specular << 31
     unsigned long long* rng;  // PCG32 state (pcg mode)
     uint32_t* seed;   // slot-indexed LCG seed (reference mode, persists across samples)
-    uint32_t* job;    // job of the entry (multi-job passes)
+    uint32_t* job;    // multi-job passes: the entry's scene << kJobBits | pass job
 };
 
 // Per-sample control block (counters cleared by k_gen).
@@ -100,9 +100,13 @@ GPR_D uint32_t npaths_of(const RenderArgs& a) {
     if constexpr (MULTI) return a.pass->npaths;
     else return a.npix;
 }
+// Multi-job queue entries carry scene << kJobBits | pass job, so the kernels
+// that only need the scene (traversal) index the scene table directly.
+constexpr uint32_t kJobBits = 10;
+static_assert(kMaxPassJobs <= (1u << kJobBits), "pass job index must fit its bits");
 template <bool MULTI>
 GPR_D SceneView scene_of(const RenderArgs& a, const QueueView& q, uint32_t i) {
-    if constexpr (MULTI) return a.scenes[a.pass->jobs[q.job[i]].j.scene];
+    if constexpr (MULTI) return a.scenes[q.job[i] >> kJobBits];
     else return a.sv;
 }
 
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(256) void k_gen(RenderArgs a) {
         q.d_tmax[tid] = make_float4(d.x, d.y, d.z, 1e34f);
         q.w_pdf[tid] = make_float4(1.f, 1.f, 1.f, 0.f);
         q.pix[tid] = fbi;
-        q.job[tid] = lo;
+        q.job[tid] = (jb.scene << kJobBits) | lo;
     } else {
         const uint32_t sample = c->sample;
         const uint32_t W = a.sv.width;
@@ -503,14 +507,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
     SceneView sv;
     if constexpr (MULTI) {
         if (valid) {
-            job = qi.job[i];
-            const PassJob& pj = a.pass->jobs[job];
+            job = qi.job[i];  // scene << kJobBits | pass job, copied to the appended entries
+            const PassJob& pj = a.pass->jobs[job & ((1u << kJobBits) - 1)];
+            sv = a.scenes[job >> kJobBits];
             // a job's paths stop at its own depth
             valid = bounce < pj.j.max_depth;
             rr_start = pj.j.rr_start;
             sample = pj.j.sample;
             lpix = (qi.pix[i] & 0x7fffffffu) - pj.j.fb_off;
-            sv = a.scenes[pj.j.scene];
         }
     } else {
         sv = a.sv;

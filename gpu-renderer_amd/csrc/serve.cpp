@@ -56,6 +56,8 @@ int arena_add(Arena& A, const gpr_scene* s, hipStream_t st, uint32_t* id) {
     if (!s || !s->committed) return fail(GPR_ERR_INVALID, "add_scene: scene not committed");
     const size_t n = s->mat.size();
     if (n == 0 || s->bvh.n != n) return fail(GPR_ERR_INVALID, "add_scene: scene has no triangles or no BVH");
+    // queue entries carry the scene index in 22 bits (scene << 10 | pass job)
+    if (A.views.size() >= (1u << 22)) return fail(GPR_ERR_INVALID, "add_scene: more than 4,194,303 scenes");
     const size_t need = scene_bytes(s);
     if (A.used + need > A.mem.bytes)
         return fail(GPR_ERR_BUDGET, "add_scene: arena full (" + std::to_string(A.used + need) + " of " +
