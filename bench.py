@@ -600,10 +600,10 @@ def main():
     graph.replay()
     torch.cuda.synchronize()
     sort_ok = bool(torch.equal(d_sk, ref_sk) and torch.equal(d_si, ref_si))
-    # three timed replays, each bracketed by device synchronisation (a replay
+    # five timed replays, each bracketed by device synchronisation (a replay
     # queued behind other work measured ~0.7 us per sort more), the median kept
     sort_replays_ms = []
-    for _ in range(3):
+    for _ in range(5):
         d_sk.zero_()
         torch.cuda.synchronize()
         e0.record()
@@ -757,7 +757,7 @@ def main():
                           "keys": N_TRI, "bytes_per_key": 68, "iters": iters,
                           "traffic": pmc_traffic("sort_1M"),
                           "timing": "mean over back-to-back sorts replayed from one hipGraph; "
-                                    "median of 3 replays, each between device synchronisations",
+                                    "median of 5 replays, each between device synchronisations",
                           "replays_ms_per_sort": [round(x, 5) for x in sort_replays_ms],
                           "graph_result_ok": sort_ok},
         "anyhit_mrays_s": round(N_RAYS / (ms_any * 1e-3) / 1e6, 2),
