@@ -233,7 +233,7 @@ GPR_D gpr_hit trace_one(const RenderArgs& a, const SceneView& sv, const Ray3& r,
 }
 
 // One triangle's shading record, read as five 16-byte loads from its line
-// (the corner uvs, read only for textured hits, are loaded by uv_at).
+// (its corner uvs, needed only for textured hits, are read by resolve_textures).
 struct Shade {
     v3 p0, p1, p2, n0, n1, n2;
     uint32_t flags, mat;
