@@ -243,6 +243,32 @@ GPR_D int trav_step(const SceneView& sv, Stack& st, Trav& t) {
         ++t.tested;
         if (ANY && t.found) return kTravDone;
     }
+    if constexpr (ANY) {
+        // any hit needs no near-to-far order (any occluder ends the walk): the
+        // lowest hit slot is visited first and the others pushed, with no
+        // sorting network
+        uint32_t m = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) m |= (hitc[c] && cha[c] >= 0) ? 1u << c : 0u;
+        ++t.fetched;
+        if (m != 0) {
+            const int c0 = __builtin_ctz(m);
+            m &= m - 1;
+            while (m != 0) {
+                const int c = 31 - __builtin_clz(m);
+                m &= ~(1u << c);
+                const int32_t ch = c == 1 ? cha[1] : c == 2 ? cha[2] : cha[3];
+                if (!st.push(make_int2(ch, 0))) return kTravOverflow;
+            }
+            t.node = c0 == 0 ? cha[0] : c0 == 1 ? cha[1] : c0 == 2 ? cha[2] : cha[3];
+            return kTravMore;
+        }
+        if (st.sp > 0) {
+            t.node = st.pop().x;
+            return kTravMore;
+        }
+        return kTravDone;
+    }
     uint32_t key[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -345,6 +371,27 @@ GPR_D int travd_step(const SceneView& sv, Stack& st, TravD& t) {
         t.pleaf[c] = cha[c];
     }
     t.pmask = leaves;
+    if constexpr (ANY) {
+        // as in trav_step: no distance order for any hit
+        uint32_t m = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) m |= (hitc[c] && cha[c] >= 0) ? 1u << c : 0u;
+        ++t.fetched;
+        if (m != 0) {
+            const int c0 = __builtin_ctz(m);
+            m &= m - 1;
+            while (m != 0) {
+                const int c = 31 - __builtin_clz(m);
+                m &= ~(1u << c);
+                const int32_t ch = c == 1 ? cha[1] : c == 2 ? cha[2] : cha[3];
+                if (!st.push(make_int2(ch, 0))) return kTravOverflow;
+            }
+            t.node = c0 == 0 ? cha[0] : c0 == 1 ? cha[1] : c0 == 2 ? cha[2] : cha[3];
+        } else {
+            t.node = -1;
+        }
+        return kTravMore;
+    }
     uint32_t key[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
