@@ -10,10 +10,46 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define GPR_HD __host__ __device__ __forceinline__
 #define GPR_D __device__ __forceinline__
 
 namespace gpr {
+
+// Load through a global-address-space pointer.  A pointer read from device
+// memory (the multi-job scene table) is generic to the compiler, which then
+// emits flat loads, counted against the LDS / scalar-memory wait counter as
+// well; every buffer the kernels read lives in global memory.
+template <class T>
+GPR_D T ldg(const T* p) {
+    if constexpr (std::is_scalar<T>::value) {
+        return *(const __attribute__((address_space(1))) T*)p;
+    } else {
+        // records: word by word (the compiler merges neighbouring words)
+        static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "word-wise global load");
+        T r;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&r);
+        const __attribute__((address_space(1))) uint32_t* src = (const __attribute__((address_space(1))) uint32_t*)p;
+#pragma unroll
+        for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = src[k];
+        return r;
+    }
+}
+// float4 / int4 are structs: load them as native 4-vectors, so the load
+// stays one global 16-byte load instead of a struct copy
+typedef float gpr_f4v __attribute__((ext_vector_type(4)));
+typedef int gpr_i4v __attribute__((ext_vector_type(4)));
+template <>
+GPR_D float4 ldg<float4>(const float4* p) {
+    const gpr_f4v v = *(const __attribute__((address_space(1))) gpr_f4v*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+template <>
+GPR_D int4 ldg<int4>(const int4* p) {
+    const gpr_i4v v = *(const __attribute__((address_space(1))) gpr_i4v*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
 
 constexpr float kPi = 3.14159265358979323846f;    // src/renderer/fwd.h:19
 constexpr float kInvPi = 0.31830988618379067154f; // src/renderer/fwd.h:20

@@ -81,13 +81,30 @@ GPR_HD int to_int_sat(float x) {
 // ------------------------------------------------------------- textures --
 // TextureView lookup (texture.h:37-52): u2 = int(uv'.x * w) % w with the
 // int converted to uint32 by the modulo, idx = v2 * w + u2.
+// Texture record k as two global 16-byte loads (the table starts on a 256-byte
+// boundary and its records are 32 bytes)
+GPR_D gpr_texture load_texture(const SceneView& sv, uint32_t k) {
+    static_assert(sizeof(gpr_texture) == 32, "texture record is two 16-byte words");
+    const float4* p = reinterpret_cast<const float4*>(sv.tex + k);
+    const float4 a = ldg(p), b = ldg(p + 1);
+    gpr_texture t;
+    t.offset = __float_as_uint(a.x);
+    t.width = __float_as_uint(a.y);
+    t.height = __float_as_uint(a.z);
+    t.pad = __float_as_uint(a.w);
+    t.uv_offset[0] = b.x;
+    t.uv_offset[1] = b.y;
+    t.uv_scale[0] = b.z;
+    t.uv_scale[1] = b.w;
+    return t;
+}
 GPR_D float4 tex_eval(const SceneView& sv, uint32_t handle, float u, float v) {
-    const gpr_texture t = sv.tex[handle - 1];
+    const gpr_texture t = load_texture(sv, handle - 1);
     const float nu = u * t.uv_scale[0] + t.uv_offset[0];
     const float nv = v * t.uv_scale[1] + t.uv_offset[1];
     const uint32_t u2 = (uint32_t)to_int_sat(nu * (float)t.width) % t.width;
     const uint32_t v2 = (uint32_t)to_int_sat(nv * (float)t.height) % t.height;
-    return sv.texels[t.offset + v2 * t.width + u2];
+    return ldg(sv.texels + (t.offset + v2 * t.width + u2));
 }
 GPR_D bool has_textures(const gpr_material& m) {
     uint32_t any = 0;
@@ -104,7 +121,7 @@ GPR_D void resolve_textures(const SceneView& sv, gpr_material& m, uint32_t t, ui
     if (flags & 2u) {
         // the corner uvs of the triangle's shading record
         const float4* r = reinterpret_cast<const float4*>(sv.shade + t);
-        const float4 q = r[5], q2 = r[6];
+        const float4 q = ldg(r + 5), q2 = ldg(r + 6);
         const float w0 = 1 - bu - bv;
         u = q.x * w0 + q.z * bu + q2.x * bv;
         v = q.y * w0 + q.w * bu + q2.y * bv;
@@ -141,10 +158,10 @@ GPR_D c3 env_le(const SceneView& sv, v3 o, v3 d) {
     float phi = (float)det_atan2((double)w.y, (double)w.x);
     if (phi < 0) phi = phi + 2 * 3.14159265358979323846f;
     const float u = phi * 0.15915494309189533577f, v = theta * 0.31830988618379067154f;
-    const gpr_texture t = sv.tex[sv.env_tex - 1];
+    const gpr_texture t = load_texture(sv, sv.env_tex - 1);
     const int iw = (int)t.width, ih = (int)t.height;
     const int u2 = min(iw - 1, to_int_sat(u * (float)iw)), v2 = min(ih - 1, to_int_sat(v * (float)ih));
-    const float4 c = sv.texels[t.offset + (uint32_t)(v2 * iw + u2)];
+    const float4 c = ldg(sv.texels + (t.offset + (uint32_t)(v2 * iw + u2)));
     return C(c.x, c.y, c.z);
 }
 

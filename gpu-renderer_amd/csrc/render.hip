@@ -106,7 +106,7 @@ constexpr uint32_t kJobBits = 10;
 static_assert(kMaxPassJobs <= (1u << kJobBits), "pass job index must fit its bits");
 template <bool MULTI>
 GPR_D SceneView scene_of(const RenderArgs& a, const QueueView& q, uint32_t i) {
-    if constexpr (MULTI) return a.scenes[q.job[i] >> kJobBits];
+    if constexpr (MULTI) return ldg(a.scenes + (q.job[i] >> kJobBits));
     else return a.sv;
 }
 
@@ -240,7 +240,7 @@ struct Shade {
 };
 GPR_D Shade load_shade(const SceneView& sv, uint32_t t) {
     const float4* r = reinterpret_cast<const float4*>(sv.shade + t);
-    const float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4];
+    const float4 a = ldg(r), b = ldg(r + 1), c = ldg(r + 2), d = ldg(r + 3), e = ldg(r + 4);
     Shade s;
     s.p0 = V(a.x, a.y, a.z);
     s.p1 = V(a.w, b.x, b.y);
@@ -265,7 +265,10 @@ GPR_D Geom setup_geom(const Shade& s, float u, float v) {
     else g.ns = norm3(cross3(s.p0 - s.p2, s.p1 - s.p2));
     return g;
 }
-GPR_D c3 emission_of(const SceneView& sv, const Shade& s) { return loadc(sv.mats[s.mat].emission); }
+GPR_D c3 emission_of(const SceneView& sv, const Shade& s) {
+    const float* e = sv.mats[s.mat].emission;
+    return C(ldg(e), ldg(e + 1), ldg(e + 2));
+}
 GPR_D float power_heur(float a, float b) {
     a *= a;
     b *= b;
@@ -508,8 +511,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
     if constexpr (MULTI) {
         if (valid) {
             job = qi.job[i];  // scene << kJobBits | pass job, copied to the appended entries
-            const PassJob& pj = a.pass->jobs[job & ((1u << kJobBits) - 1)];
-            sv = a.scenes[job >> kJobBits];
+            const PassJob pj = ldg(a.pass->jobs + (job & ((1u << kJobBits) - 1)));
+            sv = ldg(a.scenes + (job >> kJobBits));
             // a job's paths stop at its own depth
             valid = bounce < pj.j.max_depth;
             rr_start = pj.j.rr_start;
@@ -551,7 +554,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
             const Shade ts = load_shade(sv, t);
             const Geom g = setup_geom(ts, hh.z, hh.w);
             const v3 wo = -norm3(rd);
-            gpr_material m = sv.mats[ts.mat];
+            gpr_material m = ldg(sv.mats + ts.mat);
             if (sv.ntex != 0 && has_textures(m)) resolve_textures(sv, m, t, ts.flags, hh.z, hh.w);
             const Frame fr = make_frame(g.ns);
             if (bounce == 0 || spec) {
@@ -566,7 +569,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(RenderArgs a, uint32_t bo
                 const uint32_t ln = sv.nlights;
                 uint32_t lid = (uint32_t)(smp[0] * ln);
                 if (lid > ln - 1) lid = ln - 1;
-                const uint32_t lt = sv.lights[lid];
+                const uint32_t lt = ldg(sv.lights + lid);
                 const float sa = sqrtf(smp[1]);
                 const Shade ls = load_shade(sv, lt);
                 const Geom lg = setup_geom(ls, 1 - sa, smp[2] * sa);

@@ -148,7 +148,7 @@ GPR_D void tri_rec_hit(const TriRec* tris, uint32_t ntri, int32_t leaf, const Ra
                        gpr_hit& hit, bool& found) {
     if ((uint32_t)leaf >= ntri) return;
     const TriRec* t = tris + leaf;
-    const float4 a = t->a, b = t->b, c = t->c;
+    const float4 a = ldg(&t->a), b = ldg(&t->b), c = ldg(&t->c);
     float th, uh, vh;
     if (tri_hit_ref(V(a.x, a.y, a.z), V(a.w, b.x, b.y), V(b.z, b.w, c.x), r.o, r.d, r.tmin, tmax, th,
                     uh, vh)) {
@@ -210,9 +210,9 @@ GPR_D int trav_step(const SceneView& sv, Stack& st, Trav& t) {
     // ly/hy at np[2]/np[3], lz/hz at np[4]/np[5]), so no per-axis min / max.
     // A 0 * inf (ray in a slab plane) gives NaN, which fmaxf / fminf ignore:
     // that axis then does not cull the child.
-    const float4 nx = np[0 + t.sx], fx = np[1 - t.sx], ny = np[2 + t.sy], fy = np[3 - t.sy];
-    const float4 nz = np[4 + t.sz], fz = np[5 - t.sz];
-    const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+    const float4 nx = ldg(np + 0 + t.sx), fx = ldg(np + 1 - t.sx), ny = ldg(np + 2 + t.sy), fy = ldg(np + 3 - t.sy);
+    const float4 nz = ldg(np + 4 + t.sz), fz = ldg(np + 5 - t.sz);
+    const int4 ch = ldg(reinterpret_cast<const int4*>(np + 6));
     const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
     const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
     const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
@@ -345,9 +345,9 @@ GPR_D int travd_step(const SceneView& sv, Stack& st, TravD& t) {
     if ((uint32_t)t.node >= sv.nbvh4 || t.fetched > sv.nbvh4) return kTravOverflow;
     const Ray3& r = t.r;
     const float4* np = reinterpret_cast<const float4*>(sv.bvh4 + t.node);
-    const float4 nx = np[0 + t.sx], fx = np[1 - t.sx], ny = np[2 + t.sy], fy = np[3 - t.sy];
-    const float4 nz = np[4 + t.sz], fz = np[5 - t.sz];
-    const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+    const float4 nx = ldg(np + 0 + t.sx), fx = ldg(np + 1 - t.sx), ny = ldg(np + 2 + t.sy), fy = ldg(np + 3 - t.sy);
+    const float4 nz = ldg(np + 4 + t.sz), fz = ldg(np + 5 - t.sz);
+    const int4 ch = ldg(reinterpret_cast<const int4*>(np + 6));
     const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
     const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
     const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
