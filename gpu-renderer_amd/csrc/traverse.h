@@ -86,7 +86,7 @@ GPR_D gpr_hit ref_trace(const SceneView& sv, Ray3 r, uint32_t* nvisit, uint32_t*
     while (true) {
         // every node is visited at most once; both checks only trip on a malformed layout
         if ((uint32_t)cur >= sv.nref || visits > sv.nref) break;
-        const gpr_ref_node nd = sv.ref[cur];
+        const gpr_ref_node nd = ldg(sv.ref + cur);
         ++visits;
         if (ref_box(nd, r.o, r.tmin, r.tmax, inv, nx, ny, nz)) {
             if (nd.leaf & 0xffu) {
@@ -94,7 +94,9 @@ GPR_D gpr_hit ref_trace(const SceneView& sv, Ray3 r, uint32_t* nvisit, uint32_t*
                 const float* p = sv.pos + 9ull * nd.idx;
                 float t, u, v;
                 ++tests;
-                if (tri_hit_ref(load3(p), load3(p + 3), load3(p + 6), r.o, r.d, r.tmin, r.tmax, t, u, v)) {
+                const v3 p0 = V(ldg(p), ldg(p + 1), ldg(p + 2)), p1 = V(ldg(p + 3), ldg(p + 4), ldg(p + 5));
+                const v3 p2 = V(ldg(p + 6), ldg(p + 7), ldg(p + 8));
+                if (tri_hit_ref(p0, p1, p2, r.o, r.d, r.tmin, r.tmax, t, u, v)) {
                     r.tmax = t;
                     hit = gpr_hit{t, (int32_t)nd.idx, u, v};
                     if (ANY) break;
