@@ -26,13 +26,16 @@ GPR_D T ldg(const T* p) {
     if constexpr (std::is_scalar<T>::value) {
         return *(const __attribute__((address_space(1))) T*)p;
     } else {
-        // records: word by word (the compiler merges neighbouring words)
+        // records: word by word (the compiler merges neighbouring words),
+        // assembled with memcpy
         static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "word-wise global load");
-        T r;
-        uint32_t* dst = reinterpret_cast<uint32_t*>(&r);
+        static_assert(std::is_trivially_copyable<T>::value, "record load by bytes");
+        uint32_t w[sizeof(T) / 4];
         const __attribute__((address_space(1))) uint32_t* src = (const __attribute__((address_space(1))) uint32_t*)p;
 #pragma unroll
-        for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = src[k];
+        for (int k = 0; k < (int)(sizeof(T) / 4); ++k) w[k] = src[k];
+        T r;
+        __builtin_memcpy(&r, w, sizeof(T));
         return r;
     }
 }
