@@ -259,8 +259,8 @@ GPR_D int trav_step(const SceneView& sv, Stack& st, Trav& t) {
             while (m != 0) {
                 const int c = 31 - __builtin_clz(m);
                 m &= ~(1u << c);
-                const int32_t ch = c == 1 ? cha[1] : c == 2 ? cha[2] : cha[3];
-                if (!st.push(make_int2(ch, 0))) return kTravOverflow;
+                const int32_t ref = c == 1 ? cha[1] : c == 2 ? cha[2] : cha[3];
+                if (!st.push(make_int2(ref, 0))) return kTravOverflow;
             }
             t.node = c0 == 0 ? cha[0] : c0 == 1 ? cha[1] : c0 == 2 ? cha[2] : cha[3];
             return kTravMore;
@@ -385,8 +385,8 @@ GPR_D int travd_step(const SceneView& sv, Stack& st, TravD& t) {
             while (m != 0) {
                 const int c = 31 - __builtin_clz(m);
                 m &= ~(1u << c);
-                const int32_t ch = c == 1 ? cha[1] : c == 2 ? cha[2] : cha[3];
-                if (!st.push(make_int2(ch, 0))) return kTravOverflow;
+                const int32_t ref = c == 1 ? cha[1] : c == 2 ? cha[2] : cha[3];
+                if (!st.push(make_int2(ref, 0))) return kTravOverflow;
             }
             t.node = c0 == 0 ? cha[0] : c0 == 1 ? cha[1] : c0 == 2 ? cha[2] : cha[3];
         } else {
